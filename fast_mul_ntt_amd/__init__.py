@@ -1,0 +1,239 @@
+"""fast_mul_ntt_amd -- MI355X (gfx950) 3-prime NTT + CRT big-integer multiply.
+
+Python binding (ctypes) of ``libfmh.so``, whose C-ABI is declared in
+``include/fmh.h``.  The functions mirror the reference's interface
+(HJimmyK/fast_mul_ntt ``src/fast_mul.h:39,45``):
+
+* ``abs_mul64(in1, in2) -> out``  with ``len(out) == len(in1) + len(in2)``
+* ``abs_sqr64(in1) -> out``       with ``len(out) == 2 * len(in1)``
+
+over little-endian ``uint64`` limb arrays (numpy), plus device-pointer entry
+points.  The compute path is the HIP library only: if ``libfmh.so`` is
+missing this module raises instead of falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+__all__ = [
+    "LIB_PATH",
+    "HEADER_PATH",
+    "lib",
+    "declared_symbols",
+    "abs_mul64",
+    "abs_sqr64",
+    "DeviceBuffer",
+    "Stream",
+    "mul_device",
+    "sqr_device",
+    "forward_spectrum",
+    "timing_enable",
+    "timing_collect",
+    "FmhError",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmh.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "fmh.h")
+
+_u64 = ctypes.c_uint64
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_LIB = None
+
+
+class FmhError(RuntimeError):
+    pass
+
+
+def declared_symbols(header: str = HEADER_PATH) -> list[str]:
+    """Function names declared in include/fmh.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M)
+    return [n for n in names if n not in ("if", "while", "for")]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libfmh.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise FmhError(f"{LIB_PATH} is missing; build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.abs_mul64.argtypes = [_pu64, _u64, _pu64, _u64, _pu64]
+    L.abs_mul64.restype = None
+    L.abs_sqr64.argtypes = [_pu64, _u64, _pu64]
+    L.abs_sqr64.restype = None
+    L.fmh_mul_device.argtypes = [_vp, _u64, _vp, _u64, _vp, _vp]
+    L.fmh_sqr_device.argtypes = [_vp, _u64, _vp, _vp]
+    L.fmh_forward_spectrum_device.argtypes = [ctypes.c_int, _vp, _u64, _u64, _vp, _vp]
+    L.fmh_max_ntt_len.restype = _u64
+    L.fmh_init.argtypes = [ctypes.c_int]
+    L.fmh_dev_alloc.argtypes = [ctypes.POINTER(_vp), _u64]
+    L.fmh_dev_free.argtypes = [_vp]
+    L.fmh_memcpy_h2d.argtypes = [_vp, _vp, _u64, _vp]
+    L.fmh_memcpy_d2h.argtypes = [_vp, _vp, _u64, _vp]
+    L.fmh_stream_create.argtypes = [ctypes.POINTER(_vp)]
+    L.fmh_stream_destroy.argtypes = [_vp]
+    L.fmh_stream_synchronize.argtypes = [_vp]
+    L.fmh_last_error.restype = ctypes.c_char_p
+    L.fmh_timing_enable.argtypes = [ctypes.c_int]
+    L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
+    L.fmh_timing_collect.restype = _u64
+    _LIB = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().fmh_last_error()
+        raise FmhError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _as_limbs(x) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.uint64)
+    if a.ndim != 1:
+        raise ValueError("limb arrays must be one-dimensional")
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_pu64)
+
+
+def abs_mul64(in1, in2, out: np.ndarray | None = None) -> np.ndarray:
+    """out = in1 * in2 (len(in1) + len(in2) limbs), host arrays, synchronous."""
+    a, b = _as_limbs(in1), _as_limbs(in2)
+    n = a.size + b.size
+    if out is None:
+        out = np.empty(n, dtype=np.uint64)
+    elif out.dtype != np.uint64 or out.size != n or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint64 array of len(in1)+len(in2) limbs")
+    lib().abs_mul64(_ptr(a), a.size, _ptr(b), b.size, _ptr(out))
+    return out
+
+
+def abs_sqr64(in1, out: np.ndarray | None = None) -> np.ndarray:
+    """out = in1 ** 2 (2 * len(in1) limbs), host arrays, synchronous."""
+    a = _as_limbs(in1)
+    if out is None:
+        out = np.empty(2 * a.size, dtype=np.uint64)
+    elif out.dtype != np.uint64 or out.size != 2 * a.size or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint64 array of 2*len(in1) limbs")
+    lib().abs_sqr64(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+class Stream:
+    """A HIP stream owned by the library (handle usable as fmh_stream_t)."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().fmh_stream_create(ctypes.byref(h)), "fmh_stream_create")
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        _check(lib().fmh_stream_synchronize(self.handle), "fmh_stream_synchronize")
+
+    def close(self) -> None:
+        if self.handle is not None:
+            _check(lib().fmh_stream_destroy(self.handle), "fmh_stream_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DeviceBuffer:
+    """Device memory holding `n` uint64 limbs."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        p = _vp()
+        _check(lib().fmh_dev_alloc(ctypes.byref(p), max(1, self.n) * 8), "fmh_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, stream: Stream | None = None) -> "DeviceBuffer":
+        a = _as_limbs(a)
+        d = cls(a.size)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a, stream: Stream | None = None) -> None:
+        a = _as_limbs(a)
+        if a.size > self.n:
+            raise ValueError("host array larger than the device buffer")
+        if a.size:
+            _check(lib().fmh_memcpy_h2d(self.ptr, a.ctypes.data, a.size * 8,
+                                        stream.handle if stream else None), "fmh_memcpy_h2d")
+
+    def download(self, n: int | None = None, stream: Stream | None = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=np.uint64)
+        if n:
+            _check(lib().fmh_memcpy_d2h(out.ctypes.data, self.ptr, n * 8,
+                                        stream.handle if stream else None), "fmh_memcpy_d2h")
+        return out
+
+    def free(self) -> None:
+        if self.ptr is not None:
+            _check(lib().fmh_dev_free(self.ptr), "fmh_dev_free")
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
+def mul_device(a: DeviceBuffer, na: int, b: DeviceBuffer, nb: int, out: DeviceBuffer,
+               stream: Stream | None = None) -> None:
+    """Enqueue out = a[:na] * b[:nb] on `stream` (device pointers, asynchronous)."""
+    if out.n < na + nb or a.n < na or b.n < nb:
+        raise ValueError("device buffers too small")
+    _check(lib().fmh_mul_device(a.ptr, na, b.ptr, nb, out.ptr, stream.handle if stream else None),
+           "fmh_mul_device")
+
+
+def sqr_device(a: DeviceBuffer, na: int, out: DeviceBuffer, stream: Stream | None = None) -> None:
+    """Enqueue out = a[:na] ** 2 on `stream` (device pointers, asynchronous)."""
+    if out.n < 2 * na or a.n < na:
+        raise ValueError("device buffers too small")
+    _check(lib().fmh_sqr_device(a.ptr, na, out.ptr, stream.handle if stream else None),
+           "fmh_sqr_device")
+
+
+def forward_spectrum(prime: int, limbs, ntt_len: int) -> np.ndarray:
+    """Canonical forward spectrum (Montgomery form, reference order) of one operand."""
+    a = _as_limbs(limbs)
+    with DeviceBuffer.from_host(a) as d, DeviceBuffer(ntt_len) as o:
+        _check(lib().fmh_forward_spectrum_device(prime, d.ptr, a.size, ntt_len, o.ptr, None),
+               "fmh_forward_spectrum_device")
+        return o.download()
+
+
+def timing_enable(on: bool) -> None:
+    lib().fmh_timing_enable(1 if on else 0)
+
+
+def timing_collect() -> dict[str, tuple[int, float]]:
+    """{kernel name: (launches, total ms)} since the last collect."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib().fmh_timing_collect(buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out

@@ -1,0 +1,305 @@
+// fmh_api.hip -- host layer and C-ABI of libfmh.so (declared in include/fmh.h).
+//
+// State per HIP device: the twiddle tables (built once on the device, grown
+// when a longer transform arrives), one scratch workspace per stream, and a
+// private stream plus staging buffers for the host-pointer drop-in.  One
+// mutex guards the state; the drop-in entry points hold it for the whole
+// call, so concurrent callers are serialised exactly as the reference's
+// reentrant functions would behave.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/fmh.h"
+#include "fmh_internal.h"
+
+namespace {
+
+using namespace fmh;
+
+struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct DevCtx {
+    bool ready = false;
+    hipStream_t stream = nullptr;  // private stream of the host-pointer API
+    Twiddles tw;
+    std::map<hipStream_t, Buffer> ws;
+    Buffer stage_a, stage_b, stage_out;
+};
+
+constexpr int kMaxDevices = 64;
+constexpr int kMinTwiddleLog = 20;
+
+std::mutex g_mu;
+DevCtx g_ctx[kMaxDevices];
+thread_local std::string t_err;
+
+int set_err(const char* what, hipError_t e) {
+    t_err = std::string(what) + ": " + hipGetErrorString(e);
+    return (int)e == 0 ? -1 : (int)e;
+}
+int set_err(const char* what) {
+    t_err = what;
+    return -1;
+}
+
+int ceil_log2(u64 x) {
+    int l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+// Callers hold g_mu.
+int ctx_get(DevCtx** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_err("hipGetDevice", e);
+    if (dev < 0 || dev >= kMaxDevices) return set_err("device index out of range");
+    DevCtx& c = g_ctx[dev];
+    if (!c.ready) {
+        e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return set_err("hipStreamCreate", e);
+        c.ready = true;
+    }
+    *out = &c;
+    return 0;
+}
+
+int ensure_buffer(Buffer& b, size_t bytes, hipStream_t s) {
+    if (b.bytes >= bytes) return 0;
+    hipError_t e;
+    if (b.ptr) {
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return set_err("hipStreamSynchronize", e);
+        e = hipFree(b.ptr);
+        if (e != hipSuccess) return set_err("hipFree", e);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return set_err("hipMalloc", e);
+    }
+    b.bytes = want;
+    return 0;
+}
+
+int ensure_twiddles(DevCtx& c, int log_n) {
+    if (c.tw.log_t >= log_n) return 0;
+    hipError_t e = hipDeviceSynchronize();  // tables may be read by in-flight work
+    if (e != hipSuccess) return set_err("hipDeviceSynchronize", e);
+    int lt = log_n < kMinTwiddleLog ? kMinTwiddleLog : log_n;
+    if (lt > kMaxLogN) lt = kMaxLogN;
+    e = build_twiddles(c.tw, lt, c.stream);
+    if (e != hipSuccess) return set_err("build_twiddles", e);
+    return 0;
+}
+
+int check_lengths(u64 len1, u64 len2) {
+    if (len1 + len2 - 1 > (1ull << kMaxLogN)) return set_err("product longer than fmh_max_ntt_len()");
+    return 0;
+}
+
+// Enqueue a product on stream s; callers hold g_mu.
+int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
+                hipStream_t s) {
+    if (na == 0 || (!square && nb == 0)) {
+        hipError_t e = enqueue_zero(out, square ? 2 * na : na + nb, s);
+        return e == hipSuccess ? 0 : set_err("enqueue_zero", e);
+    }
+    const u64 nb_eff = square ? na : nb;
+    if (check_lengths(na, nb_eff)) return -1;
+    const u64 out_len = na + nb_eff;
+    const int log_n = ceil_log2(out_len - 1);
+    if (ensure_twiddles(c, log_n)) return -1;
+    Buffer& w = c.ws[s];
+    if (ensure_buffer(w, workspace_bytes(log_n, square, out_len), s)) return -1;
+    hipError_t e = square ? enqueue_sqr(a, na, out, w.ptr, c.tw, s)
+                          : enqueue_mul(a, na, b, nb, out, w.ptr, c.tw, s);
+    if (e != hipSuccess) return set_err("kernel launch", e);
+    return 0;
+}
+
+[[noreturn]] void die(const char* fn) {
+    fprintf(stderr, "%s: %s\n", fn, t_err.c_str());
+    abort();
+}
+
+void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 len2, bool square,
+                  u64* out) {
+    const u64 out_len = square ? 2 * len1 : len1 + len2;
+    if (len1 == 0 || (!square && len2 == 0)) {
+        if (out_len) memset(out, 0, out_len * sizeof(u64));
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) die(fn);
+    if (ensure_buffer(c->stage_a, len1 * sizeof(u64), c->stream)) die(fn);
+    if (!square && ensure_buffer(c->stage_b, len2 * sizeof(u64), c->stream)) die(fn);
+    if (ensure_buffer(c->stage_out, out_len * sizeof(u64), c->stream)) die(fn);
+    u64* da = (u64*)c->stage_a.ptr;
+    u64* db = square ? da : (u64*)c->stage_b.ptr;
+    u64* dout = (u64*)c->stage_out.ptr;
+    hipError_t e = hipMemcpyAsync(da, in1, len1 * sizeof(u64), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+        set_err("hipMemcpyAsync H2D", e);
+        die(fn);
+    }
+    if (!square) {
+        e = hipMemcpyAsync(db, in2, len2 * sizeof(u64), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            set_err("hipMemcpyAsync H2D", e);
+            die(fn);
+        }
+    }
+    if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, c->stream)) die(fn);
+    e = hipMemcpyAsync(out, dout, out_len * sizeof(u64), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) {
+        set_err("hipMemcpyAsync D2H", e);
+        die(fn);
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        set_err("hipStreamSynchronize", e);
+        die(fn);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void abs_mul64(const u64* in1, u64 len1, const u64* in2, u64 len2, u64* out) {
+    host_product("abs_mul64", in1, len1, in2, len2, false, out);
+}
+
+void abs_sqr64(const u64* in1, u64 len1, u64* out) {
+    host_product("abs_sqr64", in1, len1, in1, len1, true, out);
+}
+
+int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
+                   fmh_stream_t stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    return run_product(*c, d_in1, len1, d_in2, len2, false, d_out, (hipStream_t)stream);
+}
+
+int fmh_sqr_device(const u64* d_in1, u64 len1, u64* d_out, fmh_stream_t stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    return run_product(*c, d_in1, len1, d_in1, len1, true, d_out, (hipStream_t)stream);
+}
+
+int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len, u64* d_out,
+                                fmh_stream_t stream) {
+    if (prime < 0 || prime > 2) return set_err("prime index must be 0, 1 or 2");
+    if (ntt_len == 0 || (ntt_len & (ntt_len - 1)) || ntt_len < len)
+        return set_err("ntt_len must be a power of two >= len");
+    const int log_n = ceil_log2(ntt_len);
+    if (log_n > kMaxLogN) return set_err("ntt_len above fmh_max_ntt_len()");
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    if (ensure_twiddles(*c, log_n)) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    Buffer& w = c->ws[s];
+    if (ensure_buffer(w, 3 * ntt_len * sizeof(u64), s)) return -1;
+    hipError_t e = enqueue_spectrum(prime, d_in, len, log_n, d_out, w.ptr, c->tw, s);
+    if (e != hipSuccess) return set_err("kernel launch", e);
+    return 0;
+}
+
+u64 fmh_max_ntt_len(void) { return 1ull << kMaxLogN; }
+
+int fmh_init(int device) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return set_err("hipSetDevice", e);
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    return ensure_twiddles(*c, kMinTwiddleLog);
+}
+
+int fmh_dev_alloc(void** ptr, u64 bytes) {
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    return e == hipSuccess ? 0 : set_err("hipMalloc", e);
+}
+
+int fmh_dev_free(void* ptr) {
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? 0 : set_err("hipFree", e);
+}
+
+int fmh_memcpy_h2d(void* dst, const void* src, u64 bytes, fmh_stream_t stream) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? 0 : set_err("hipMemcpy H2D", e);
+}
+
+int fmh_memcpy_d2h(void* dst, const void* src, u64 bytes, fmh_stream_t stream) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? 0 : set_err("hipMemcpy D2H", e);
+}
+
+int fmh_stream_create(fmh_stream_t* stream) {
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return set_err("hipStreamCreate", e);
+    *stream = (fmh_stream_t)s;
+    return 0;
+}
+
+int fmh_stream_destroy(fmh_stream_t stream) {
+    hipStream_t s = (hipStream_t)stream;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDevices) {
+            auto it = g_ctx[dev].ws.find(s);
+            if (it != g_ctx[dev].ws.end()) {
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(it->second.ptr);
+                g_ctx[dev].ws.erase(it);
+            }
+        }
+    }
+    hipError_t e = hipStreamDestroy(s);
+    return e == hipSuccess ? 0 : set_err("hipStreamDestroy", e);
+}
+
+int fmh_stream_synchronize(fmh_stream_t stream) {
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? 0 : set_err("hipStreamSynchronize", e);
+}
+
+const char* fmh_last_error(void) { return t_err.c_str(); }
+
+int fmh_timing_enable(int on) {
+    timing_enable(on != 0);
+    return 0;
+}
+
+u64 fmh_timing_collect(char* buf, u64 buflen) {
+    const std::string s = timing_collect();
+    if (buf && buflen) {
+        const size_t n = s.size() < buflen - 1 ? s.size() : (size_t)(buflen - 1);
+        memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (u64)s.size() + 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,931 @@
+// fmh_ntt.hip -- gfx950 kernels of the 3-prime NTT multiply and their
+// launchers.
+//
+// Transform.  Every transform is the in-place radix-2 decimation-in-
+// frequency NTT of length N = 2^n (stage s pairs x[i] and x[i + h] with
+// h = 2^(n-1-s) and multiplies the difference by w_{2h}^(i mod 2h)).  Its
+// output at position j is X[bitrev_n(j)], the order of the reference's dif_i
+// and of its split-radix recursion (src/core.h:443-474,
+// src/fast_mul.c:74-98).  The inverse runs the Cooley-Tukey stages in the
+// opposite order with inverse twiddles (src/core.h:476-509); the factor N it
+// leaves is removed in the CRT constants.
+//
+// Passes.  The n stages are cut into consecutive groups.  A group [s0, s1)
+// only combines indices that differ in bits n-s1 .. n-s0-1, so a workgroup
+// loads one "line" per (block, offset) pair: 2^(s1-s0) words at stride
+// B = 2^(n-s1), a few adjacent lines at once so every row of the tile is one
+// coalesced segment.  The group's stages then run in LDS as radix-2^2 rounds
+// (two stages per LDS round trip, the reference's radix-2^2 pairing).  The
+// last group (stride 1) is the row pass; for products it is fused with the
+// pointwise product and the first inverse stages, so the spectra never
+// travel to HBM.
+//
+// Twiddles.  One table per prime and direction, tw[h + t] = w_{2h}^t in
+// Montgomery form (the layout of the reference's NTTshort, src/core.h:
+// 163-170, 214-238, but canonical and covering every h < 2^log_t); a stage
+// of half-span h at global index i reads tw[h + (i mod h)].
+//
+// Data representation.  Residues are plain values x mod p kept in [0, 2p)
+// (see fmh_field.h); multiplying by a Montgomery-form twiddle keeps them
+// plain.  The pointwise Montgomery product leaves a factor 2^-64 that, like
+// N, is folded into the CRT constants.
+#include <stdio.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fmh_internal.h"
+
+namespace fmh {
+
+// ------------------------------------------------------------------ host math
+
+typedef unsigned __int128 u128;
+
+static u64 h_powmod(u64 b, u64 e, u64 p) {
+    u128 r = 1, x = b % p;
+    while (e) {
+        if (e & 1) r = (r * x) % p;
+        x = (x * x) % p;
+        e >>= 1;
+    }
+    return (u64)r;
+}
+static u64 h_mont(u64 x, u64 p) { return (u64)((((u128)(x % p)) << 64) % p); }
+
+static const u64 kP[3] = {Prime<0>::p, Prime<1>::p, Prime<2>::p};
+static const u64 kG[3] = {Prime<0>::g, Prime<1>::g, Prime<2>::g};
+
+CrtConsts crt_consts(int log_n) {
+    CrtConsts c;
+    for (int i = 0; i < 3; i++) {
+        const u64 p = kP[i];
+        const u64 mi = (u64)(((u128)kP[(i + 1) % 3] * kP[(i + 2) % 3]) % p);
+        const u64 mi_inv = h_powmod(mi, p - 2, p);
+        const u64 n_inv = h_powmod(h_powmod(2, (u64)log_n, p), p - 2, p);
+        const u64 r = (u64)((((u128)1) << 64) % p);
+        const u64 r2 = (u64)(((u128)r * r) % p);
+        c.k[i] = (u64)(((u128)((u128)n_inv * r2 % p) * mi_inv) % p);
+    }
+    return c;
+}
+
+Plan make_plan(int log_n) {
+    Plan pl{};
+    pl.log_n = log_n;
+    if (log_n <= kRowTileLog) {
+        pl.n_col = 0;
+        pl.row_s = 0;
+        return pl;
+    }
+    pl.row_s = log_n - kRowMaxLogL;
+    const int r = pl.row_s;
+    const int ncol = (r + kColMaxLogL - 1) / kColMaxLogL;
+    pl.n_col = ncol;
+    int s = 0;
+    for (int i = 0; i < ncol; i++) {
+        pl.col_s[i] = s;
+        s += (r - s) / (ncol - i);
+    }
+    pl.col_s[ncol] = r;
+    return pl;
+}
+
+static int ceil_log2(u64 x) {
+    int l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+size_t workspace_bytes(int log_n, bool square, u64 out_len) {
+    const u64 n = 1ull << log_n;
+    const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
+    return (square ? 3 : 6) * n * sizeof(u64) + (1 + chunks) * sizeof(u32) + 256;
+}
+
+// ---------------------------------------------------------------- timing
+
+static std::mutex g_tmu;
+static bool g_timing = false;
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t a, b;
+};
+static std::vector<TimedLaunch> g_launches;
+
+void timing_enable(bool on) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing = on;
+}
+bool timing_enabled() {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    return g_timing;
+}
+hipError_t timing_begin(hipStream_t s, hipEvent_t* start) {
+    *start = nullptr;
+    if (!timing_enabled()) return hipSuccess;
+    hipError_t e = hipEventCreate(start);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(*start, s);
+}
+hipError_t timing_end(const char* name, hipStream_t s, hipEvent_t start) {
+    if (start == nullptr) return hipSuccess;
+    hipEvent_t stop;
+    hipError_t e = hipEventCreate(&stop);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(stop, s);
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_launches.push_back({name, start, stop});
+    return e;
+}
+
+// Sums the recorded launches per kernel name; returns them as
+// "name count total_ms" lines and forgets them.
+std::string timing_collect() {
+    std::vector<TimedLaunch> v;
+    {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        v.swap(g_launches);
+    }
+    std::vector<std::pair<std::string, std::pair<long, double>>> agg;
+    for (auto& t : v) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(t.b);
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+        bool found = false;
+        for (auto& a : agg)
+            if (a.first == t.name) {
+                a.second.first++;
+                a.second.second += ms;
+                found = true;
+            }
+        if (!found) agg.push_back({t.name, {1L, (double)ms}});
+    }
+    std::string out;
+    char line[256];
+    for (auto& a : agg) {
+        snprintf(line, sizeof(line), "%s %ld %.6f\n", a.first.c_str(), a.second.first, a.second.second);
+        out += line;
+    }
+    return out;
+}
+
+#define FMH_LAUNCH(name, kern, grid, block, lds, stream, ...)                    \
+    do {                                                                         \
+        hipEvent_t ev_;                                                          \
+        hipError_t e_ = timing_begin(stream, &ev_);                              \
+        if (e_ != hipSuccess) return e_;                                         \
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);         \
+        e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess) return e_;                                         \
+        e_ = timing_end(name, stream, ev_);                                      \
+        if (e_ != hipSuccess) return e_;                                         \
+    } while (0)
+
+// ---------------------------------------------------------------- LDS tiles
+//
+// A tile holds C = 2^logC lines of L = 2^logL words.
+//   MAP 0 (column tile): word (c, j) at t[(j << logC) + c]; line c has stride
+//          offset b = b0 + c within the global stride B = 2^logB.
+//   MAP 1 (row tile):    word (c, j) at t[(c << logL) + j]; stride B = 1.
+// For a stage of local half-span hl the global half-span is H = hl * B and
+// the twiddle of the lower element at local j is tw[H + (j mod hl) B + b].
+
+template <int P, int MAP, int NT>
+__device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__ t1, int logL,
+                                         int logC, int logB, u32 b0,
+                                         const u64* __restrict__ W) {
+    int rem = logL;
+    while (rem >= 2) {
+        const int lq = rem - 2;
+        const int hl = 1 << lq;
+        const int nbf = 1 << (logL - 2 + logC);
+        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+            int c, u, blk;
+            if (MAP == 0) {
+                c = beta & ((1 << logC) - 1);
+                const int r = beta >> logC;
+                u = r & (hl - 1);
+                blk = r >> lq;
+            } else {
+                u = beta & (hl - 1);
+                const int r = beta >> lq;
+                blk = r & ((1 << (logL - 2 - lq)) - 1);
+                c = r >> (logL - 2 - lq);
+            }
+            const int j0 = (blk << (lq + 2)) + u;
+            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
+            const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
+            const u64 wc = W[((u32)(hl + u) << logB) + bb];
+            int i0, st;
+            if (MAP == 0) {
+                i0 = (j0 << logC) + c;
+                st = hl << logC;
+            } else {
+                i0 = (c << logL) + j0;
+                st = hl;
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+                u64* t = (q == 0) ? t0 : t1;
+                u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
+                bf_dif<P>(x0, x2, wa);
+                bf_dif<P>(x1, x3, wb);
+                bf_dif<P>(x0, x1, wc);
+                bf_dif<P>(x2, x3, wc);
+                t[i0] = x0;
+                t[i0 + st] = x1;
+                t[i0 + 2 * st] = x2;
+                t[i0 + 3 * st] = x3;
+            }
+        }
+        __syncthreads();
+        rem -= 2;
+    }
+    if (rem == 1) {
+        const int nbf = 1 << (logL - 1 + logC);
+        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+            int c, blk;
+            if (MAP == 0) {
+                c = beta & ((1 << logC) - 1);
+                blk = beta >> logC;
+            } else {
+                blk = beta & ((1 << (logL - 1)) - 1);
+                c = beta >> (logL - 1);
+            }
+            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const u64 w = W[(1u << logB) + bb];
+            int i0, st;
+            if (MAP == 0) {
+                i0 = ((2 * blk) << logC) + c;
+                st = 1 << logC;
+            } else {
+                i0 = (c << logL) + 2 * blk;
+                st = 1;
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+                u64* t = (q == 0) ? t0 : t1;
+                u64 x0 = t[i0], x1 = t[i0 + st];
+                bf_dif<P>(x0, x1, w);
+                t[i0] = x0;
+                t[i0 + st] = x1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int P, int MAP>
+__device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
+                                         const u64* __restrict__ W) {
+    if (logL & 1) {
+        const int nbf = 1 << (logL - 1 + logC);
+        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+            int c, blk;
+            if (MAP == 0) {
+                c = beta & ((1 << logC) - 1);
+                blk = beta >> logC;
+            } else {
+                blk = beta & ((1 << (logL - 1)) - 1);
+                c = beta >> (logL - 1);
+            }
+            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const u64 w = W[(1u << logB) + bb];
+            int i0, st;
+            if (MAP == 0) {
+                i0 = ((2 * blk) << logC) + c;
+                st = 1 << logC;
+            } else {
+                i0 = (c << logL) + 2 * blk;
+                st = 1;
+            }
+            u64 x0 = t[i0], x1 = t[i0 + st];
+            bf_dit<P>(x0, x1, w);
+            t[i0] = x0;
+            t[i0 + st] = x1;
+        }
+        __syncthreads();
+    }
+    for (int lq = logL & 1; lq + 2 <= logL; lq += 2) {
+        const int hl = 1 << lq;
+        const int nbf = 1 << (logL - 2 + logC);
+        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+            int c, u, blk;
+            if (MAP == 0) {
+                c = beta & ((1 << logC) - 1);
+                const int r = beta >> logC;
+                u = r & (hl - 1);
+                blk = r >> lq;
+            } else {
+                u = beta & (hl - 1);
+                const int r = beta >> lq;
+                blk = r & ((1 << (logL - 2 - lq)) - 1);
+                c = r >> (logL - 2 - lq);
+            }
+            const int j0 = (blk << (lq + 2)) + u;
+            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
+            const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
+            const u64 wc = W[((u32)(hl + u) << logB) + bb];
+            int i0, st;
+            if (MAP == 0) {
+                i0 = (j0 << logC) + c;
+                st = hl << logC;
+            } else {
+                i0 = (c << logL) + j0;
+                st = hl;
+            }
+            u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
+            bf_dit<P>(x0, x1, wc);
+            bf_dit<P>(x2, x3, wc);
+            bf_dit<P>(x0, x2, wa);
+            bf_dit<P>(x1, x3, wb);
+            t[i0] = x0;
+            t[i0 + st] = x1;
+            t[i0 + 2 * st] = x2;
+            t[i0 + 3 * st] = x3;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ column pass
+
+struct ColArgs {
+    const u64* limbs[2];
+    u64 len[2];
+    u64* res[6];  // [operand * 3 + prime]
+    const u64* tw[3];
+    int log_n, s0, s1;
+    int from_limbs;
+};
+
+template <int P, bool FWD>
+__device__ void col_body(const ColArgs& A, int op) {
+    extern __shared__ u64 lds[];
+    const int logL = A.s1 - A.s0;
+    const int logB = A.log_n - A.s1;
+    const int logC = kColLogC;
+    const int C = 1 << logC;
+    const int n = C << logL;
+    const u64 lam0 = (u64)blockIdx.x << logC;
+    const u64 a = lam0 >> logB;
+    const u32 b0 = (u32)(lam0 & ((1ull << logB) - 1));
+    const u64 base = (a << (logL + logB)) + b0;
+    u64* __restrict__ res = A.res[op * 3 + P];
+    const u64* __restrict__ limbs = A.limbs[op];
+    const u64 len = A.len[op];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int j = e >> logC, c = e & (C - 1);
+        const u64 g = base + ((u64)j << logB) + (u64)c;
+        u64 v;
+        if (A.from_limbs)
+            v = g < len ? reduce_word<P>(limbs[g]) : 0ull;
+        else
+            v = res[g];
+        lds[e] = v;
+    }
+    __syncthreads();
+    if (FWD)
+        tile_dif<P, 0, 1>(lds, nullptr, logL, logC, logB, b0, A.tw[P]);
+    else
+        tile_dit<P, 0>(lds, logL, logC, logB, b0, A.tw[P]);
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int j = e >> logC, c = e & (C - 1);
+        const u64 g = base + ((u64)j << logB) + (u64)c;
+        res[g] = lds[e];
+    }
+}
+
+template <bool FWD>
+__global__ __launch_bounds__(kThreads) void k_col(ColArgs A) {
+    const int op = blockIdx.z;
+    switch (blockIdx.y) {
+        case 0: col_body<0, FWD>(A, op); break;
+        case 1: col_body<1, FWD>(A, op); break;
+        default: col_body<2, FWD>(A, op); break;
+    }
+}
+
+// --------------------------------------------------------------- row pass
+
+enum RowMode { kRowMul = 0, kRowSqr = 1, kRowFwdOnly = 2 };
+
+struct RowArgs {
+    const u64* limbs[2];
+    u64 len[2];
+    u64* res[6];
+    const u64* tw[3];
+    const u64* itw[3];
+    u64 r2[3];  // 2^128 mod p (spectrum export to Montgomery form)
+    int log_n, s0;
+    int from_limbs;
+    int mode;
+};
+
+template <int P>
+__device__ void row_body(const RowArgs& A) {
+    extern __shared__ u64 lds[];
+    const int logL = A.log_n - A.s0;
+    const int logC = min(A.s0, kRowTileLog - logL);
+    const int n = 1 << (logL + logC);
+    const u64 base = (u64)blockIdx.x << (logL + logC);
+    const bool two = (A.mode == kRowMul);
+    u64* __restrict__ tA = lds;
+    u64* __restrict__ tB = lds + n;
+    u64* __restrict__ rA = A.res[P];
+    const u64* __restrict__ rB = A.res[3 + P];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const u64 g = base + (u64)e;
+        if (A.from_limbs) {
+            tA[e] = g < A.len[0] ? reduce_word<P>(A.limbs[0][g]) : 0ull;
+            if (two) tB[e] = g < A.len[1] ? reduce_word<P>(A.limbs[1][g]) : 0ull;
+        } else {
+            tA[e] = rA[g];
+            if (two) tB[e] = rB[g];
+        }
+    }
+    __syncthreads();
+    if (two)
+        tile_dif<P, 1, 2>(tA, tB, logL, logC, 0, 0u, A.tw[P]);
+    else
+        tile_dif<P, 1, 1>(tA, nullptr, logL, logC, 0, 0u, A.tw[P]);
+    if (A.mode == kRowFwdOnly) {
+        const u64 r2 = A.r2[P];
+        for (int e = threadIdx.x; e < n; e += blockDim.x) rA[base + e] = canon<P>(redc<P>(tA[e], r2));
+        return;
+    }
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const u64 x = tA[e];
+        tA[e] = redc<P>(x, two ? tB[e] : x);
+    }
+    __syncthreads();
+    tile_dit<P, 1>(tA, logL, logC, 0, 0u, A.itw[P]);
+    for (int e = threadIdx.x; e < n; e += blockDim.x) rA[base + e] = tA[e];
+}
+
+__global__ __launch_bounds__(kThreads) void k_row(RowArgs A) {
+    switch (blockIdx.y) {
+        case 0: row_body<0>(A); break;
+        case 1: row_body<1>(A); break;
+        default: row_body<2>(A); break;
+    }
+}
+
+// ------------------------------------------------------- CRT + carry scan
+//
+// Coefficient k of the product is c_k = CRT(x_0, x_1, x_2) in [0, P) with
+// x_i = redc(res_i[k], K_i) (src/core.h:77-104 computes the same value).
+// The limbs are out_k = (s_k + carry_k) mod 2^64 with
+// s_k = c_k[0] + c_{k-1}[1] + c_{k-2}[2] < 3 * 2^64 and carry_k <= 2
+// (c_k < 2^184).  The carry out of limb k is a map f_k: {0..3} -> {0..3},
+// f_k(c) = hi(s_k) + [lo(s_k) + c >= 2^64]; maps compose associatively, so
+// the serial loop of src/fast_mul.c:378-389 becomes a scan over maps with a
+// single-pass decoupled look-back between chunks.  A map is packed as four
+// 2-bit values, m(c) = (m >> 2c) & 3.
+
+struct CarryArgs {
+    const u64* res[3];
+    u64 conv_len, out_len;
+    u64* out;
+    u32* flags;  // [0] chunk counter, [1 + t] status of chunk t
+    u64 K[3];
+    u64 M[3][2];  // P / p_i as two words
+    u64 Pw[3];    // P = p0 p1 p2 as three words
+};
+
+__device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
+    u32 r = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const u32 fc = (f >> (2 * c)) & 3u;
+        r |= ((g >> (2 * fc)) & 3u) << (2 * c);
+    }
+    return r;
+}
+__device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (2 * c)) & 3u; }
+constexpr u32 kMapId = 0xE4u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
+
+__device__ __forceinline__ void crt_coef(const CarryArgs& A, u64 k, u64& w0, u64& w1, u64& w2) {
+    w0 = w1 = w2 = 0;
+    if (k >= A.conv_len) return;
+    const u64 x0 = canon<0>(redc<0>(A.res[0][k], A.K[0]));
+    const u64 x1 = canon<1>(redc<1>(A.res[1][k], A.K[1]));
+    const u64 x2 = canon<2>(redc<2>(A.res[2][k], A.K[2]));
+    const u64 xs[3] = {x0, x1, x2};
+    u128 acc_lo = 0;  // bits 0..127
+    u64 acc_hi = 0;   // bits 128..191
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const u128 lo = (u128)xs[i] * A.M[i][0];
+        const u128 hi = (u128)xs[i] * A.M[i][1];
+        // term = lo + hi * 2^64
+        const u128 mid = (lo >> 64) + (u128)(u64)hi;
+        const u128 low128 = (u128)(u64)lo | (mid << 64);
+        const u64 top = (u64)(hi >> 64) + (u64)(mid >> 64);
+        const u128 s = acc_lo + low128;
+        acc_hi += top + (s < acc_lo ? 1ull : 0ull);
+        acc_lo = s;
+    }
+    w0 = (u64)acc_lo;
+    w1 = (u64)(acc_lo >> 64);
+    w2 = acc_hi;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        // if (w >= P) w -= P
+        bool ge;
+        if (w2 != A.Pw[2])
+            ge = w2 > A.Pw[2];
+        else if (w1 != A.Pw[1])
+            ge = w1 > A.Pw[1];
+        else
+            ge = w0 >= A.Pw[0];
+        if (ge) {
+            const u64 b0 = w0 < A.Pw[0] ? 1ull : 0ull;
+            w0 -= A.Pw[0];
+            const u128 d1 = (u128)w1 - A.Pw[1] - b0;
+            w1 = (u64)d1;
+            const u64 b1 = (u64)(d1 >> 64) & 1ull;
+            w2 = w2 - A.Pw[2] - b1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_crt_carry(CarryArgs A) {
+    constexpr int E = kCarryPerThread;
+    constexpr int CH = kCarryChunk;
+    __shared__ u64 s_w1[CH + 2];
+    __shared__ u64 s_w2[CH + 2];
+    __shared__ u32 s_wave[kThreads / 64];
+    __shared__ u32 s_tile;
+    __shared__ u32 s_cin;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(&A.flags[0], 1u);
+    __syncthreads();
+    const u64 tile = s_tile;
+    const u64 k0 = tile * (u64)CH;
+
+    u64 lo0[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const u64 k = k0 + (u64)(tid * E + e);
+        u64 w0, w1, w2;
+        crt_coef(A, k, w0, w1, w2);
+        lo0[e] = w0;
+        s_w1[tid * E + e + 2] = w1;
+        s_w2[tid * E + e + 2] = w2;
+    }
+    if (tid < 2) {
+        u64 w0 = 0, w1 = 0, w2 = 0;
+        if (k0 >= (u64)(2 - tid)) crt_coef(A, k0 - 2 + tid, w0, w1, w2);
+        s_w1[tid] = w1;
+        s_w2[tid] = w2;
+    }
+    __syncthreads();
+
+    u64 lo[E];
+    u32 f[E];
+    u32 F = kMapId;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = tid * E + e + 2;
+        const u128 s = (u128)lo0[e] + s_w1[i - 1] + s_w2[i - 2];
+        const u64 l = (u64)s;
+        const u32 h = (u32)(s >> 64);
+        lo[e] = l;
+        const u32 f0 = h;
+        const u32 f1 = h + (l == ~0ull ? 1u : 0u);
+        const u32 f2 = h + (l >= ~0ull - 1 ? 1u : 0u);
+        const u32 f3 = h + (l >= ~0ull - 2 ? 1u : 0u);
+        f[e] = f0 | (f1 << 2) | (f2 << 4) | (f3 << 6);
+        F = map_compose(f[e], F);
+    }
+
+    // Block-wide exclusive scan of the per-thread maps (order matters).
+    const int lane = tid & 63, wave = tid >> 6;
+    u32 incl = F;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl = map_compose(incl, o);
+    }
+    u32 excl = __shfl_up(incl, 1, 64);
+    if (lane == 0) excl = kMapId;
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    u32 wpre = kMapId;
+    for (int w = 0; w < wave; w++) wpre = map_compose(s_wave[w], wpre);
+    excl = map_compose(excl, wpre);
+
+    if (tid == 0) {
+        u32 agg = kMapId;
+        for (int w = 0; w < kThreads / 64; w++) agg = map_compose(s_wave[w], agg);
+        u32* fl = A.flags + 1;
+        u32 cin = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&fl[0], (2u << 8) | map_apply(agg, 0), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&fl[tile], (1u << 8) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            u32 acc = kMapId;
+            long long j = (long long)tile - 1;
+            while (true) {
+                const u32 v = __hip_atomic_load(&fl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const u32 st = v >> 8;
+                if (st == 0) continue;
+                if (st == 2) {
+                    cin = map_apply(acc, v & 3u);
+                    break;
+                }
+                acc = map_compose(acc, v & 0xFFu);
+                j--;
+            }
+            __hip_atomic_store(&fl[tile], (2u << 8) | map_apply(agg, cin), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_cin = cin;
+    }
+    __syncthreads();
+    u32 c = map_apply(excl, s_cin);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const u64 k = k0 + (u64)(tid * E + e);
+        if (k < A.out_len) A.out[k] = lo[e] + c;
+        c = map_apply(f[e], c);
+    }
+}
+
+// ---------------------------------------------------------- small kernels
+
+__global__ void k_fill_u32(u32* p, u64 n, u32 v) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(u64* p, u64 n, u64 v) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Twiddle tables.  Top section: tw[T/2 + t] = root^t for t < T/2, computed
+// by square-and-multiply (canonical Montgomery products); lower sections are
+// subsamples: tw[h + t] = tw[T/2 + t * (T / 2h)].
+template <int P>
+__device__ u64 mont_pow_canon(u64 base_m, u64 e, u64 one_m) {
+    u64 r = one_m;
+    while (e) {
+        if (e & 1) r = canon<P>(redc<P>(r, base_m));
+        base_m = canon<P>(redc<P>(base_m, base_m));
+        e >>= 1;
+    }
+    return r;
+}
+
+struct TwArgs {
+    u64* tab[6];   // [dir * 3 + prime]
+    u64 root[6];   // Montgomery root of order T (dir 0) or its inverse (dir 1)
+    u64 one[3];
+    int log_t;
+};
+
+__global__ void k_tw_top(TwArgs A) {
+    const u64 half = 1ull << (A.log_t - 1);
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = blockIdx.y;  // dir * 3 + prime
+    if (t >= half) return;
+    const int P = idx % 3;
+    u64 v;
+    switch (P) {
+        case 0: v = mont_pow_canon<0>(A.root[idx], t, A.one[0]); break;
+        case 1: v = mont_pow_canon<1>(A.root[idx], t, A.one[1]); break;
+        default: v = mont_pow_canon<2>(A.root[idx], t, A.one[2]); break;
+    }
+    A.tab[idx][half + t] = v;
+}
+
+__global__ void k_tw_low(TwArgs A) {
+    const u64 half = 1ull << (A.log_t - 1);
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;  // 0 .. half-1
+    const int idx = blockIdx.y;
+    if (i >= half) return;
+    u64* tab = A.tab[idx];
+    if (i == 0) {
+        tab[0] = 0;
+        return;
+    }
+    const int lh = 63 - __clzll((long long)i);  // h = 2^lh <= i < 2h
+    const u64 h = 1ull << lh;
+    const u64 t = i - h;
+    tab[i] = tab[half + t * (half >> lh)];
+}
+
+hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
+    if (log_t < 1) log_t = 1;
+    const u64 T = 1ull << log_t;
+    for (int i = 0; i < 3; i++) {
+        if (tw.fwd[i]) (void)hipFree(tw.fwd[i]);
+        if (tw.inv[i]) (void)hipFree(tw.inv[i]);
+        tw.fwd[i] = tw.inv[i] = nullptr;
+    }
+    tw.log_t = -1;
+    TwArgs A{};
+    A.log_t = log_t;
+    for (int i = 0; i < 3; i++) {
+        hipError_t e = hipMalloc(&tw.fwd[i], T * sizeof(u64));
+        if (e != hipSuccess) return e;
+        e = hipMalloc(&tw.inv[i], T * sizeof(u64));
+        if (e != hipSuccess) return e;
+        const u64 p = kP[i];
+        const u64 root = h_powmod(kG[i], (p - 1) >> log_t, p);
+        const u64 iroot = h_powmod(root, p - 2, p);
+        A.tab[i] = tw.fwd[i];
+        A.tab[3 + i] = tw.inv[i];
+        A.root[i] = h_mont(root, p);
+        A.root[3 + i] = h_mont(iroot, p);
+        A.one[i] = h_mont(1, p);
+    }
+    const u64 half = T / 2;
+    dim3 grid((unsigned)((half + 255) / 256), 6);
+    hipLaunchKernelGGL(k_tw_top, grid, dim3(256), 0, s, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tw_low, grid, dim3(256), 0, s, A);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    tw.log_t = log_t;
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------- pipelines
+
+static void fill_carry_consts(CarryArgs& C, int log_n) {
+    const CrtConsts k = crt_consts(log_n);
+    for (int i = 0; i < 3; i++) {
+        C.K[i] = k.k[i];
+        const u128 m = (u128)kP[(i + 1) % 3] * kP[(i + 2) % 3];
+        C.M[i][0] = (u64)m;
+        C.M[i][1] = (u64)(m >> 64);
+    }
+    const u128 m01 = (u128)kP[0] * kP[1];
+    const u128 lo = (u128)(u64)m01 * kP[2];
+    const u128 hi = (u128)(u64)(m01 >> 64) * kP[2];
+    C.Pw[0] = (u64)lo;
+    const u128 mid = (lo >> 64) + (u64)hi;
+    C.Pw[1] = (u64)mid;
+    C.Pw[2] = (u64)(hi >> 64) + (u64)(mid >> 64);
+}
+
+static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
+                                  void* ws, const Twiddles& tw, hipStream_t s) {
+    const u64 out_len = na + nb;
+    const u64 conv_len = out_len - 1;
+    const int log_n = ceil_log2(conv_len);
+    if (log_n > kMaxLogN || log_n > tw.log_t) return hipErrorInvalidValue;
+    const u64 N = 1ull << log_n;
+    const Plan pl = make_plan(log_n);
+    u64* res = (u64*)ws;
+    u32* flags = (u32*)(res + (square ? 3 : 6) * N);
+    const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
+
+    FMH_LAUNCH("fill_flags", k_fill_u32, dim3((unsigned)((chunks + 1 + 255) / 256)), dim3(256), 0, s,
+               flags, chunks + 1, 0u);
+
+    u64* rp[6];
+    for (int i = 0; i < 6; i++) rp[i] = res + (u64)i * N;
+
+    const int nops = square ? 1 : 2;
+    for (int pi = 0; pi < pl.n_col; pi++) {
+        ColArgs C{};
+        C.limbs[0] = a;
+        C.limbs[1] = square ? a : b;
+        C.len[0] = na;
+        C.len[1] = square ? na : nb;
+        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
+        C.log_n = log_n;
+        C.s0 = pl.col_s[pi];
+        C.s1 = pl.col_s[pi + 1];
+        C.from_limbs = (pi == 0);
+        const int logL = C.s1 - C.s0;
+        const u64 tiles = N >> (logL + kColLogC);
+        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, nops), dim3(kThreads), lds, s, C);
+    }
+    {
+        RowArgs R{};
+        R.limbs[0] = a;
+        R.limbs[1] = b;
+        R.len[0] = na;
+        R.len[1] = nb;
+        for (int i = 0; i < 6; i++) R.res[i] = rp[i];
+        for (int i = 0; i < 3; i++) {
+            R.tw[i] = tw.fwd[i];
+            R.itw[i] = tw.inv[i];
+        }
+        R.log_n = log_n;
+        R.s0 = pl.row_s;
+        R.from_limbs = (pl.n_col == 0);
+        R.mode = square ? kRowSqr : kRowMul;
+        const int logL = log_n - pl.row_s;
+        const int logC = std::min(pl.row_s, kRowTileLog - logL);
+        const u64 tiles = N >> (logL + logC);
+        const size_t lds = (size_t)(square ? 1 : 2) * ((size_t)1 << (logL + logC)) * sizeof(u64);
+        FMH_LAUNCH(square ? "row_sqr" : "row_mul", k_row, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds,
+                   s, R);
+    }
+    for (int pi = pl.n_col - 1; pi >= 0; pi--) {
+        ColArgs C{};
+        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        for (int i = 0; i < 3; i++) C.tw[i] = tw.inv[i];
+        C.log_n = log_n;
+        C.s0 = pl.col_s[pi];
+        C.s1 = pl.col_s[pi + 1];
+        C.from_limbs = 0;
+        const int logL = C.s1 - C.s0;
+        const u64 tiles = N >> (logL + kColLogC);
+        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
+        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, C);
+    }
+    {
+        CarryArgs K{};
+        for (int i = 0; i < 3; i++) K.res[i] = rp[i];
+        K.conv_len = conv_len;
+        K.out_len = out_len;
+        K.out = out;
+        K.flags = flags;
+        fill_carry_consts(K, log_n);
+        FMH_LAUNCH("crt_carry", k_crt_carry, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
+    }
+    return hipSuccess;
+}
+
+hipError_t enqueue_mul(const u64* a, u64 na, const u64* b, u64 nb, u64* out, void* ws, const Twiddles& tw,
+                       hipStream_t s) {
+    return enqueue_product(a, na, b, nb, false, out, ws, tw, s);
+}
+
+hipError_t enqueue_sqr(const u64* a, u64 na, u64* out, void* ws, const Twiddles& tw, hipStream_t s) {
+    return enqueue_product(a, na, a, na, true, out, ws, tw, s);
+}
+
+hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    FMH_LAUNCH("fill_out", k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, 0ull);
+    return hipSuccess;
+}
+
+hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
+                            const Twiddles& tw, hipStream_t s) {
+    if (log_n > kMaxLogN || log_n > tw.log_t || prime < 0 || prime > 2) return hipErrorInvalidValue;
+    const u64 N = 1ull << log_n;
+    const Plan pl = make_plan(log_n);
+    u64* res = (u64*)ws;
+    u64* rp[6];
+    for (int i = 0; i < 3; i++) rp[i] = rp[3 + i] = res + (u64)i * N;
+    for (int pi = 0; pi < pl.n_col; pi++) {
+        ColArgs C{};
+        C.limbs[0] = C.limbs[1] = a;
+        C.len[0] = C.len[1] = na;
+        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
+        C.log_n = log_n;
+        C.s0 = pl.col_s[pi];
+        C.s1 = pl.col_s[pi + 1];
+        C.from_limbs = (pi == 0);
+        const int logL = C.s1 - C.s0;
+        const u64 tiles = N >> (logL + kColLogC);
+        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, C);
+    }
+    RowArgs R{};
+    R.limbs[0] = R.limbs[1] = a;
+    R.len[0] = R.len[1] = na;
+    for (int i = 0; i < 6; i++) R.res[i] = rp[i];
+    for (int i = 0; i < 3; i++) {
+        R.tw[i] = tw.fwd[i];
+        R.itw[i] = tw.inv[i];
+        const u64 p = kP[i];
+        const u64 r = (u64)((((u128)1) << 64) % p);
+        R.r2[i] = (u64)(((u128)r * r) % p);
+    }
+    R.log_n = log_n;
+    R.s0 = pl.row_s;
+    R.from_limbs = (pl.n_col == 0);
+    R.mode = kRowFwdOnly;
+    const int logL = log_n - pl.row_s;
+    const int logC = std::min(pl.row_s, kRowTileLog - logL);
+    const u64 tiles = N >> (logL + logC);
+    const size_t lds = ((size_t)1 << (logL + logC)) * sizeof(u64);
+    FMH_LAUNCH("row_fwd", k_row, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, R);
+    return hipMemcpyAsync(out, rp[prime], N * sizeof(u64), hipMemcpyDeviceToDevice, s);
+}
+
+}  // namespace fmh

@@ -1,0 +1,83 @@
+/*
+ * fmh.h -- C-ABI of libfmh.so, the MI355X (gfx950) 3-prime NTT + CRT
+ * big-integer multiply.
+ *
+ * Numbers are little-endian arrays of 64-bit limbs.  Outputs are exactly the
+ * limbs of the product (len1 + len2 limbs, 2 * len1 for a square); the top
+ * limb may be zero.  Results are bit-identical to the reference's src/
+ * path: the exact product is unique.
+ *
+ * Every call is thread-safe.  Functions returning int give 0 on success and
+ * a nonzero code on failure (fmh_last_error() describes the last failure of
+ * the calling thread).
+ */
+#ifndef FMH_H
+#define FMH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef unsigned long long u64;
+
+/* ---------------------------------------------------------------- drop-in
+ * Replace src/fast_mul.h:39 and src/fast_mul.h:45 of HJimmyK/fast_mul_ntt
+ * (implemented in src/fast_mul.c:308-396 and 398-460): same names, same
+ * signatures, host pointers, synchronous.  `out` may alias an input.  Like
+ * the reference, a device failure prints a message to stderr and calls
+ * abort().  A zero-length operand gives the all-zero product (the reference
+ * aborts or overruns its buffers there).                                    */
+void abs_mul64(const u64* in1, u64 len1, const u64* in2, u64 len2, u64* out);
+void abs_sqr64(const u64* in1, u64 len1, u64* out);
+
+/* ------------------------------------------------------ device pointers
+ * No counterpart in the reference (its API is host-only).  Inputs and output
+ * live in device memory of the current HIP device; the work is enqueued on
+ * `stream` (a hipStream_t; NULL is the null stream) and returns before it
+ * completes.  Scratch memory is owned by the library, one workspace per
+ * stream: calls on one stream run in order, calls on different streams do
+ * not share scratch.  `out` must not alias an input.                       */
+typedef void* fmh_stream_t;
+
+int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
+                   fmh_stream_t stream);
+int fmh_sqr_device(const u64* d_in1, u64 len1, u64* d_out, fmh_stream_t stream);
+
+/* Forward transform of one operand for prime index `prime` (0, 1, 2 = the
+ * reference's moduli 1, 2, 3), zero-padded to ntt_len (a power of two >=
+ * len): ntt_len canonical residues X * 2^64 mod p in the reference's order,
+ * position j = X[bitrev(j)] -- the values src/core.h:443-474 dif_i (and the
+ * split-radix step of src/fast_mul.c:74-95 above 2^17 points) produce, reduced
+ * to [0, p).  For parity checks.                                          */
+int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len, u64* d_out,
+                                fmh_stream_t stream);
+
+/* Largest supported transform length (products with len1 + len2 - 1 up to
+ * this many coefficients). */
+u64 fmh_max_ntt_len(void);
+
+/* -------------------------------------------------------------- plumbing
+ * Thin wrappers over the HIP runtime so that C, Go (cgo) or Python (ctypes)
+ * callers can drive the device entry points without HIP headers.          */
+int fmh_init(int device);
+int fmh_dev_alloc(void** ptr, u64 bytes);
+int fmh_dev_free(void* ptr);
+int fmh_memcpy_h2d(void* dst, const void* src, u64 bytes, fmh_stream_t stream);
+int fmh_memcpy_d2h(void* dst, const void* src, u64 bytes, fmh_stream_t stream);
+int fmh_stream_create(fmh_stream_t* stream);
+int fmh_stream_destroy(fmh_stream_t stream);
+int fmh_stream_synchronize(fmh_stream_t stream);
+const char* fmh_last_error(void);
+
+/* Per-kernel timing with HIP events recorded on each kernel's launch stream
+ * (for benchmarks).  fmh_timing_collect writes "name count total_ms" lines
+ * for the launches recorded since the last collect and returns the number of
+ * bytes needed (including the terminating NUL). */
+int fmh_timing_enable(int on);
+u64 fmh_timing_collect(char* buf, u64 buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMH_H */

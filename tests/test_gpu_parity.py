@@ -1,0 +1,129 @@
+"""GPU parity: libfmh.so (through its C-ABI) against the CPU oracle.
+
+The oracle restates the reference's src/ algorithm and is itself pinned
+(tests/test_oracle.py) against the reference's compiled core.h, its second
+library fft_small, schoolbook products and closed forms.  Products are
+integers, so the bar is bit-exact equality.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# Sizes of the reference's own multiply test (fft_small/fsd_test.c:313-314),
+# plus lengths around every transform and pass boundary of this build.
+SIZES = [1, 2, 3, 5, 17, 33, 64, 100, 255, 256, 257, 300, 511, 512, 513, 700, 1024, 1100,
+         1500, 2048, 2500, 2900]
+
+
+def test_small_products_all_pairs(fm, orc):
+    seed = 7
+    for i, an in enumerate(SIZES):
+        for bn in SIZES[i::3]:
+            a = orc.splitmix(seed, an)
+            b = orc.splitmix(seed + 1, bn)
+            seed += 2
+            want = orc.schoolbook(a, b)
+            got = fm.abs_mul64(a, b)
+            assert np.array_equal(got, want), (an, bn)
+            got = fm.abs_mul64(b, a)
+            assert np.array_equal(got, want), ("swapped", an, bn)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 2047, 2048, 2049, 4096, 5000])
+def test_small_squares(fm, orc, n):
+    a = orc.splitmix(100 + n, n)
+    assert np.array_equal(fm.abs_sqr64(a), orc.schoolbook(a, a))
+
+
+# Lengths whose N = 2^n crosses every plan change: single row pass (n <= 12),
+# one column pass (13..21), two column passes (22..).
+@pytest.mark.parametrize("n1,n2", [(2048, 2049), (4096, 4096), (3000, 9000), (65536, 65536),
+                                   (100000, 31), (262144, 262144), (1 << 20, 1 << 20)])
+def test_products_vs_oracle(fm, orc, n1, n2):
+    a = orc.splitmix(n1 * 3 + 1, n1)
+    b = orc.splitmix(n2 * 5 + 2, n2)
+    want = orc.mul(a, b)
+    got = fm.abs_mul64(a, b)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [4097, 65536, 1 << 20])
+def test_squares_vs_oracle(fm, orc, n):
+    a = orc.splitmix(n + 77, n)
+    assert np.array_equal(fm.abs_sqr64(a), orc.sqr(a))
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (7, 3), (3000, 3000), (70000, 1000), (1 << 20, 1 << 20)])
+def test_all_ones_worst_carry(fm, n1, n2):
+    """(B^a - 1)(B^b - 1) = B^(a+b) - B^a - B^b + 1 with B = 2^64: carry chains of length ~n."""
+    a = np.full(n1, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    b = np.full(n2, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    got = fm.abs_mul64(a, b)
+    lo, hi = min(n1, n2), max(n1, n2)
+    want = np.zeros(n1 + n2, dtype=np.uint64)
+    # value = B^(n1+n2) - B^n1 - B^n2 + 1 computed limb-wise
+    from oracle.oracle import limbs_to_int, int_to_limbs
+    if n1 + n2 <= 20000:
+        v = (1 << (64 * (n1 + n2))) - (1 << (64 * n1)) - (1 << (64 * n2)) + 1
+        want = int_to_limbs(v, n1 + n2)
+        assert np.array_equal(got, want)
+    else:
+        # closed form by limbs: 1, 0.., then 0xFF..FE at index lo, 0xFF.. up to hi-1 (if lo<hi),
+        # then 0xFF..FF.. with the top limb pattern; check via structure
+        w = np.zeros(n1 + n2, dtype=np.uint64)
+        w[0] = 1
+        if lo == hi:
+            w[lo] = 0xFFFFFFFFFFFFFFFE
+            w[lo + 1:] = 0xFFFFFFFFFFFFFFFF
+        else:
+            w[lo:hi] = 0xFFFFFFFFFFFFFFFF
+            w[lo] = 0xFFFFFFFFFFFFFFFF
+            w[hi] = 0xFFFFFFFFFFFFFFFE
+            w[hi + 1:] = 0xFFFFFFFFFFFFFFFF
+        assert np.array_equal(got, w)
+
+
+def test_zero_length_operands(fm):
+    a = np.arange(1, 6, dtype=np.uint64)
+    assert np.array_equal(fm.abs_mul64(a, np.zeros(0, np.uint64)), np.zeros(5, np.uint64))
+    assert np.array_equal(fm.abs_mul64(np.zeros(0, np.uint64), a), np.zeros(5, np.uint64))
+    assert fm.abs_sqr64(np.zeros(0, np.uint64)).size == 0
+
+
+def test_output_aliases_input(fm, orc):
+    """out may alias in1 (the reference consumes its inputs before writing)."""
+    import ctypes
+    a = orc.splitmix(5, 1000)
+    b = orc.splitmix(6, 1000)
+    want = orc.mul(a, b)
+    buf = np.zeros(2000, dtype=np.uint64)
+    buf[:1000] = a
+    p = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    fm.lib().abs_mul64(p, 1000, b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 1000, p)
+    assert np.array_equal(buf, want)
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 2, 3, 4, 5, 8, 11, 12, 13, 17, 18, 21])
+def test_forward_spectrum_vs_oracle(fm, orc, log_n):
+    n = 1 << log_n
+    for prime in range(3):
+        limbs = orc.splitmix(31 * log_n + prime, max(1, n // 2))
+        got = fm.forward_spectrum(prime, limbs, n)
+        want = orc.forward_spectrum(prime, limbs, n)
+        assert np.array_equal(got, want), (prime, log_n)
+
+
+def test_device_entry_points(fm, orc):
+    a = orc.splitmix(91, 50000)
+    b = orc.splitmix(92, 70001)
+    want = orc.mul(a, b)
+    with fm.Stream() as s, fm.DeviceBuffer.from_host(a, s) as da, fm.DeviceBuffer.from_host(b, s) as db, \
+            fm.DeviceBuffer(a.size + b.size) as do:
+        fm.mul_device(da, a.size, db, b.size, do, s)
+        s.synchronize()
+        assert np.array_equal(do.download(), want)
+        fm.sqr_device(da, a.size, do, s)
+        s.synchronize()
+        assert np.array_equal(do.download(2 * a.size), orc.sqr(a))
