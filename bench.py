@@ -99,6 +99,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus})",
@@ -134,6 +136,11 @@ def main() -> int:
     kt = fm.timing_collect()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     got = dout.download(N1 + N2)
+    if args.copy_calib:
+        words = 1 << 25
+        with fm.DeviceBuffer(words) as cs, fm.DeviceBuffer(words) as cd:
+            fm.copy_device(cd, cs, words, stream)
+            stream.synchronize()
 
     # Host-pointer drop-in (PCIe included), reported beside the device number.
     t_host = []

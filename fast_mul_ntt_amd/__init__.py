@@ -32,6 +32,7 @@ __all__ = [
     "mul_device",
     "sqr_device",
     "forward_spectrum",
+    "copy_device",
     "timing_enable",
     "timing_collect",
     "FmhError",
@@ -83,6 +84,7 @@ def lib() -> ctypes.CDLL:
     L.fmh_stream_create.argtypes = [ctypes.POINTER(_vp)]
     L.fmh_stream_destroy.argtypes = [_vp]
     L.fmh_stream_synchronize.argtypes = [_vp]
+    L.fmh_copy_device.argtypes = [_vp, _vp, _u64, _vp]
     L.fmh_last_error.restype = ctypes.c_char_p
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
     L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
@@ -222,6 +224,14 @@ def forward_spectrum(prime: int, limbs, ntt_len: int) -> np.ndarray:
         _check(lib().fmh_forward_spectrum_device(prime, d.ptr, a.size, ntt_len, o.ptr, None),
                "fmh_forward_spectrum_device")
         return o.download()
+
+
+def copy_device(dst: DeviceBuffer, src: DeviceBuffer, n: int, stream: Stream | None = None) -> None:
+    """Enqueue a plain n-word device copy (kernel "copy")."""
+    if dst.n < n or src.n < n:
+        raise ValueError("device buffers too small")
+    _check(lib().fmh_copy_device(dst.ptr, src.ptr, n, stream.handle if stream else None),
+           "fmh_copy_device")
 
 
 def timing_enable(on: bool) -> None:

@@ -223,6 +223,11 @@ int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len
 
 u64 fmh_max_ntt_len(void) { return 1ull << kMaxLogN; }
 
+int fmh_copy_device(u64* d_dst, const u64* d_src, u64 n, fmh_stream_t stream) {
+    hipError_t e = enqueue_copy(d_dst, d_src, n, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : set_err("kernel launch", e);
+}
+
 int fmh_init(int device) {
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return set_err("hipSetDevice", e);

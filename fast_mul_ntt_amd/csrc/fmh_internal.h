@@ -63,6 +63,8 @@ hipError_t enqueue_mul(const u64* a, u64 na, const u64* b, u64 nb, u64* out, voi
 hipError_t enqueue_sqr(const u64* a, u64 na, u64* out, void* ws, const Twiddles& tw,
                        hipStream_t s);
 hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s);
+// Plain device copy of n words (HBM calibration kernel for profilers).
+hipError_t enqueue_copy(u64* dst, const u64* src, u64 n, hipStream_t s);
 // Forward spectrum of one operand for one prime, canonical Montgomery form,
 // reference (bit-reversed) order.  ws holds at least 2^log_n words.
 hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,

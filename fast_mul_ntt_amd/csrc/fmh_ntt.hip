@@ -876,6 +876,19 @@ hipError_t enqueue_sqr(const u64* a, u64 na, u64* out, void* ws, const Twiddles&
     return enqueue_product(a, na, a, na, true, out, ws, tw, s);
 }
 
+__global__ void k_copy(u64* __restrict__ dst, const u64* __restrict__ src, u64 n) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t enqueue_copy(u64* dst, const u64* src, u64 n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    u64 blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    FMH_LAUNCH("copy", k_copy, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    return hipSuccess;
+}
+
 hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     FMH_LAUNCH("fill_out", k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, 0ull);

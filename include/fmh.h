@@ -67,6 +67,9 @@ int fmh_memcpy_d2h(void* dst, const void* src, u64 bytes, fmh_stream_t stream);
 int fmh_stream_create(fmh_stream_t* stream);
 int fmh_stream_destroy(fmh_stream_t stream);
 int fmh_stream_synchronize(fmh_stream_t stream);
+/* Plain word copy on the device, launched as kernel "copy": the known-size
+ * reference kernel against which profiler byte counters are calibrated. */
+int fmh_copy_device(u64* d_dst, const u64* d_src, u64 n, fmh_stream_t stream);
 const char* fmh_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on each kernel's launch stream
