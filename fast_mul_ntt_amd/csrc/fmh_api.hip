@@ -215,7 +215,7 @@ int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len
     if (ensure_twiddles(*c, log_n)) return -1;
     hipStream_t s = (hipStream_t)stream;
     Buffer& w = c->ws[s];
-    if (ensure_buffer(w, 3 * ntt_len * sizeof(u64), s)) return -1;
+    if (ensure_buffer(w, workspace_bytes(log_n, true, 0), s)) return -1;
     hipError_t e = enqueue_spectrum(prime, d_in, len, log_n, d_out, w.ptr, c->tw, s);
     if (e != hipSuccess) return set_err("kernel launch", e);
     return 0;

@@ -1,27 +1,36 @@
-// fmh_field.h -- arithmetic modulo the three NTT primes, for gfx950 device
-// code and for the host planner.
+// fmh_field.h -- arithmetic modulo the three NTT primes, shared by the gfx950
+// kernels and by host code (the CPU checks of this arithmetic compile the
+// same functions with g++).
 //
 // The primes, roots and R = 2^64 are those of src/data.h:69-81 in the
-// reference; every prime has the special form p = c * 2^k + 1:
+// reference; every prime has the form p = c * 2^k + 1 with k >= 55:
 //   p0 = 69 * 2^55 + 1   g = 5
 //   p1 = 27 * 2^56 + 1   g = 5
 //   p2 = 29 * 2^57 + 1   g = 3
-// so -p^-1 mod 2^64 = p - 2 (src/data.h:106-108) and 4p < 2^64.
+// so p = PHI * 2^32 + 1 with PHI = p >> 32, and 4p < 2^64.
 //
-// Value ranges used by the kernels (all proofs are one line):
-//   * redc(x, w) = x * w * 2^-64 mod p lies in [0, 2p) for any 64-bit x when
-//     w < p, because (x*w + m*p) / 2^64 < (2^64 * p + 2^64 * p) / 2^64.
-//   * redc(x, y) lies in [0, 2p) when x, y < 2p, because 4p^2 < 2^64 * p.
-//   * add2p(x, y) of x, y in [0, 2p) lies in [0, 2p); x - y + 2p lies in
-//     (0, 4p) and fits 64 bits since 4p < 2^64.
-// Data words between stages therefore always lie in [0, 2p); twiddles are
-// canonical Montgomery forms (w * 2^64 mod p, in [0, p)).
+// Montgomery reduction (R = 2^64) is done one 32-bit word at a time: for
+// p = 1 (mod 2^32) the per-word quotient digit is simply m = -x_0 mod 2^32,
+// so each word costs one 32x32+64 multiply-add with PHI.  The result is the
+// same integer as the reference's REDC (src/macro.h:109-118): both equal
+// (x + m p) / 2^64 for the unique m < 2^64 with x + m p = 0 (mod 2^64).
+//
+// Value ranges used by the kernels:
+//   * redc(x, y) < 2p whenever x * y < p * 2^64, because the result is below
+//     x*y/2^64 + p.  This holds for any 64-bit x with y < p (a canonical
+//     twiddle), and for x, y < 2p (4p^2 < 2^64 p).
+//   * forward (DIF) data stay in [0, 2p): (x + y) is reduced once, and
+//     x - y + 2p lies in (0, 4p), which fits 64 bits.
+//   * inverse (DIT) data stay in [0, 4p) (Harvey's lazy butterfly): the upper
+//     input is reduced below 2p, the product below 2p, so both outputs lie
+//     below 4p < 2^64.
 #pragma once
 
 #include <stdint.h>
 
 typedef unsigned long long u64;
 typedef unsigned int u32;
+typedef unsigned char u8;
 
 #if defined(__HIPCC__)
 #define FMH_HD __host__ __device__ __forceinline__
@@ -62,45 +71,62 @@ static_assert(Prime<1>::p == (Prime<1>::c << Prime<1>::k) + 1, "p1 form");
 static_assert(Prime<2>::p == (Prime<2>::c << Prime<2>::k) + 1, "p2 form");
 static_assert(Prime<2>::p < (~0ull) / 4, "4p < 2^64 for the largest prime");
 
-#if defined(__HIPCC__)
-
-// Montgomery reduction of the 128-bit product a*b: (a*b + m*p) / 2^64 with
-// m = (a*b mod 2^64) * (p - 2) mod 2^64.  The low words of a*b and m*p sum to
-// 0 or 2^64, so the carry into the high word is (lo != 0).
+// (a * b + m p) / 2^64 with m = -a b p^-1 mod 2^64, by two word steps.
 template <int P>
-__device__ __forceinline__ u64 redc(u64 a, u64 b) {
-    constexpr u64 p = Prime<P>::p;
-    const u64 lo = a * b;
-    const u64 hi = __umul64hi(a, b);
-    const u64 m = lo * (p - 2);
-    const u64 mh = __umul64hi(m, p);
-    return hi + mh + (lo != 0 ? 1ull : 0ull);
+FMH_HD u64 redc(u64 a, u64 b) {
+    constexpr u32 PHI = (u32)(Prime<P>::p >> 32);
+    static_assert(((u64)PHI << 32) + 1 == Prime<P>::p, "p = PHI 2^32 + 1");
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    // x = a b = hi 2^64 + x1 2^32 + x0
+    const u64 p00 = (u64)a0 * b0;
+    const u64 p01 = (u64)a0 * b1 + (p00 >> 32);
+    const u64 p10 = (u64)a1 * b0 + (u32)p01;
+    const u64 hi = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    const u32 x0 = (u32)p00, x1 = (u32)p10;
+    // word 0: m0 = -x0; (x + m0 p) / 2^32 = hi 2^32 + T
+    const u32 m0 = 0u - x0;
+    const u64 T = (u64)m0 * PHI + x1 + (x0 != 0 ? 1u : 0u);
+    // word 1: m1 = -T0; result = hi + T / 2^32 + m1 PHI + [T0 != 0]
+    const u32 y0 = (u32)T;
+    const u32 m1 = 0u - y0;
+    return (u64)m1 * PHI + hi + (T >> 32) + (y0 != 0 ? 1u : 0u);
 }
 
+// 64 x 64 -> 128-bit product from 32-bit multiply-adds.
+FMH_HD void mul_64x64(u64 a, u64 b, u64& lo, u64& hi) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    const u64 p01 = (u64)a0 * b1 + (p00 >> 32);
+    const u64 p10 = (u64)a1 * b0 + (u32)p01;
+    hi = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    lo = (p10 << 32) | (u32)p00;
+}
+
+// x - 2p if x >= 2p (x < 4p): the unsigned minimum of x and x - 2p.
 template <int P>
-__device__ __forceinline__ u64 add2p(u64 a, u64 b) {
+FMH_HD u64 norm2p(u64 x) {
     constexpr u64 p2 = 2 * Prime<P>::p;
-    const u64 s = a + b;
-    return s >= p2 ? s - p2 : s;
+    const u64 d = x - p2;
+    return d < x ? d : x;
 }
 
 template <int P>
-__device__ __forceinline__ u64 norm2p(u64 a) {
-    constexpr u64 p2 = 2 * Prime<P>::p;
-    return a >= p2 ? a - p2 : a;
+FMH_HD u64 add2p(u64 a, u64 b) {
+    return norm2p<P>(a + b);
 }
 
 template <int P>
-__device__ __forceinline__ u64 canon(u64 a) {  // [0, 2p) -> [0, p)
+FMH_HD u64 canon(u64 a) {  // [0, 2p) -> [0, p)
     constexpr u64 p = Prime<P>::p;
-    return a >= p ? a - p : a;
+    const u64 d = a - p;
+    return d < a ? d : a;
 }
 
 // Any 64-bit word x -> x mod p in [0, p), using p = c 2^k + 1:
 // with q = floor((x >> k) / c), x - q p = ((x>>k) - q c) 2^k + (x mod 2^k) - q
 // lies in [-q, p - 2]; a negative result is fixed by adding p.
 template <int P>
-__device__ __forceinline__ u64 reduce_word(u64 x) {
+FMH_HD u64 reduce_word(u64 x) {
     constexpr u64 p = Prime<P>::p;
     const u32 xh = (u32)(x >> Prime<P>::k);
     const u32 q = xh / (u32)Prime<P>::c;
@@ -108,24 +134,24 @@ __device__ __forceinline__ u64 reduce_word(u64 x) {
     return ((long long)r < 0) ? r + p : r;
 }
 
-// Gentleman-Sande (DIF) butterfly: (x, y) -> (x + y, (x - y) w).
+// Gentleman-Sande (DIF) butterfly, x, y in [0, 2p): (x + y, (x - y) w).
 template <int P>
-__device__ __forceinline__ void bf_dif(u64& x, u64& y, u64 w) {
+FMH_HD void bf_dif(u64& x, u64& y, u64 w) {
     constexpr u64 p2 = 2 * Prime<P>::p;
     const u64 s = add2p<P>(x, y);
-    y = redc<P>(x - y + p2, w);
+    y = redc<P>(x + p2 - y, w);
     x = s;
 }
 
-// Cooley-Tukey (DIT) butterfly: (x, y) -> (x + y w, x - y w).
+// Cooley-Tukey (DIT) butterfly, x, y in [0, 4p): (x + y w, x - y w), outputs
+// in [0, 4p).
 template <int P>
-__device__ __forceinline__ void bf_dit(u64& x, u64& y, u64 w) {
+FMH_HD void bf_dit(u64& x, u64& y, u64 w) {
     constexpr u64 p2 = 2 * Prime<P>::p;
+    const u64 xr = norm2p<P>(x);
     const u64 t = redc<P>(y, w);
-    y = norm2p<P>(x - t + p2);
-    x = add2p<P>(x, t);
+    x = xr + t;
+    y = xr + p2 - t;
 }
-
-#endif  // __HIPCC__
 
 }  // namespace fmh

@@ -14,14 +14,24 @@ namespace fmh {
 // 2^27 + 2^27 limbs).  The reference's own limit is 2^55 (src/data.h:69).
 constexpr int kMaxLogN = 28;
 
-// LDS tile geometry.
-constexpr int kThreads = 256;        // threads per workgroup for every pass
-constexpr int kColLogC = 3;          // column passes: 8 adjacent lines
-constexpr int kColMaxLogL = 10;      // column passes: at most 2^10 points/line
-constexpr int kRowMaxLogL = 11;      // row pass: at most 2^11 points/line
-constexpr int kRowTileLog = 12;      // row pass: 2^12 points per operand tile
-constexpr int kCarryPerThread = 8;   // limbs per thread in the carry scan
+constexpr int kThreads = 256;         // threads per workgroup, CRT and carry kernels
+constexpr int kMaxPassThreads = 1024; // launch bound of the NTT pass kernels
+constexpr int kCarryPerThread = 4;    // limbs per thread in the carry scan
 constexpr int kCarryChunk = kThreads * kCarryPerThread;
+
+// Tile geometry of the NTT passes.  The defaults can be overridden for tuning
+// sweeps with FMH_TUNE="col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,
+// col_threads=512,row_threads=512" (read once per process).
+struct Tuning {
+    int col_logc = 3;       // column passes: 2^col_logc adjacent lines per tile
+    int col_maxl = 10;      // column passes: at most 2^col_maxl points per line
+    int row_maxl = 11;      // row pass: at most 2^row_maxl points per line
+    int row_tile = 12;      // row pass: 2^row_tile points per operand tile; a
+                            // transform of at most 2^row_tile points is one pass
+    int col_threads = 512;  // threads per workgroup, column passes
+    int row_threads = 512;  // threads per workgroup, row pass
+};
+const Tuning& tuning();
 
 // One transform plan: stages [0, n) of the in-place radix-2 DIF split into
 // column passes [s_i, s_{i+1}) and a final row pass [s_row, n).
@@ -34,9 +44,10 @@ struct Plan {
 
 Plan make_plan(int log_n);
 
-// Per-call CRT constants: K_i = N^-1 * R^2 * (P/p_i)^-1 mod p_i.
+// Per-call CRT constants (Garner form, see crt_consts in fmh_ntt.hip).
 struct CrtConsts {
-    u64 k[3];
+    u64 k00, k11, k10, k22, k20, k21;
+    u64 m01[2];  // p0 * p1 as two words
 };
 CrtConsts crt_consts(int log_n);
 

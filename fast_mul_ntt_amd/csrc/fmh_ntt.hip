@@ -30,6 +30,7 @@
 // plain.  The pointwise Montgomery product leaves a factor 2^-64 that, like
 // N, is folded into the CRT constants.
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <mutex>
@@ -59,31 +60,87 @@ static u64 h_mont(u64 x, u64 p) { return (u64)((((u128)(x % p)) << 64) % p); }
 static const u64 kP[3] = {Prime<0>::p, Prime<1>::p, Prime<2>::p};
 static const u64 kG[3] = {Prime<0>::g, Prime<1>::g, Prime<2>::g};
 
+// Garner constants of the CRT for transform length 2^log_n.  The inverse
+// transform leaves v_i = C * 2^log_n * 2^-64 (mod p_i) for coefficient C; with
+// redc(v, K) = v K 2^-64:
+//   x0 = C mod p0                        = redc(v0, k00)
+//   y1 = (C - x0) / p0 mod p1            = redc(v1, k11) - redc(x0, k10)
+//   y2 = (C - x0 - p0 y1) / (p0 p1) mod p2
+//                                        = redc(v2, k22) - redc(x0, k20) - redc(y1, k21)
+//   C  = x0 + p0 y1 + p0 p1 y2           (< p0 p1 p2, no final reduction)
 CrtConsts crt_consts(int log_n) {
-    CrtConsts c;
+    auto inv = [](u64 a, u64 p) { return h_powmod(a % p, p - 2, p); };
+    auto mulm = [](u64 a, u64 b, u64 p) { return (u64)(((u128)a * b) % p); };
+    u64 ninv[3], r[3], r2[3];
     for (int i = 0; i < 3; i++) {
-        const u64 p = kP[i];
-        const u64 mi = (u64)(((u128)kP[(i + 1) % 3] * kP[(i + 2) % 3]) % p);
-        const u64 mi_inv = h_powmod(mi, p - 2, p);
-        const u64 n_inv = h_powmod(h_powmod(2, (u64)log_n, p), p - 2, p);
-        const u64 r = (u64)((((u128)1) << 64) % p);
-        const u64 r2 = (u64)(((u128)r * r) % p);
-        c.k[i] = (u64)(((u128)((u128)n_inv * r2 % p) * mi_inv) % p);
+        ninv[i] = inv(h_powmod(2, (u64)log_n, kP[i]), kP[i]);
+        r[i] = (u64)((((u128)1) << 64) % kP[i]);
+        r2[i] = mulm(r[i], r[i], kP[i]);
     }
+    const u64 p0 = kP[0], p1 = kP[1], p2 = kP[2];
+    const u64 p01_mod2 = (u64)(((u128)p0 * p1) % p2);
+    CrtConsts c;
+    c.k00 = mulm(ninv[0], r2[0], p0);
+    c.k11 = mulm(mulm(ninv[1], r2[1], p1), inv(p0, p1), p1);
+    c.k10 = mulm(inv(p0, p1), r[1], p1);
+    c.k22 = mulm(mulm(ninv[2], r2[2], p2), inv(p01_mod2, p2), p2);
+    c.k20 = mulm(inv(p01_mod2, p2), r[2], p2);
+    c.k21 = mulm(inv(p1, p2), r[2], p2);
+    const u128 m01 = (u128)p0 * p1;
+    c.m01[0] = (u64)m01;
+    c.m01[1] = (u64)(m01 >> 64);
     return c;
 }
 
+const Tuning& tuning() {
+    static Tuning t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char* env = getenv("FMH_TUNE");
+        if (!env) return;
+        std::string spec(env);
+        size_t pos = 0;
+        while (pos < spec.size()) {
+            size_t end = spec.find(',', pos);
+            if (end == std::string::npos) end = spec.size();
+            const std::string kv = spec.substr(pos, end - pos);
+            const size_t eq = kv.find('=');
+            if (eq != std::string::npos) {
+                const std::string k = kv.substr(0, eq);
+                const int v = atoi(kv.c_str() + eq + 1);
+                if (k == "col_logc") t.col_logc = v;
+                if (k == "col_maxl") t.col_maxl = v;
+                if (k == "row_maxl") t.row_maxl = v;
+                if (k == "row_tile") t.row_tile = v;
+                if (k == "col_threads") t.col_threads = v;
+                if (k == "row_threads") t.row_threads = v;
+            }
+            pos = end + 1;
+        }
+        // keep the geometry inside what the kernels assume
+        t.col_logc = std::max(0, std::min(t.col_logc, 5));
+        t.col_maxl = std::max(1, std::min(t.col_maxl, 13));
+        t.row_tile = std::max(2, std::min(t.row_tile, 13));
+        t.row_maxl = std::max(t.col_logc, std::min(t.row_maxl, t.row_tile));
+        if (t.col_maxl + t.col_logc > 14) t.col_maxl = 14 - t.col_logc;  // 128 KiB of LDS
+        t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
+        t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
+    });
+    return t;
+}
+
 Plan make_plan(int log_n) {
+    const Tuning& tu = tuning();
     Plan pl{};
     pl.log_n = log_n;
-    if (log_n <= kRowTileLog) {
+    if (log_n <= tu.row_tile) {
         pl.n_col = 0;
         pl.row_s = 0;
         return pl;
     }
-    pl.row_s = log_n - kRowMaxLogL;
+    pl.row_s = log_n - tu.row_maxl;
     const int r = pl.row_s;
-    const int ncol = (r + kColMaxLogL - 1) / kColMaxLogL;
+    const int ncol = (r + tu.col_maxl - 1) / tu.col_maxl;
     pl.n_col = ncol;
     int s = 0;
     for (int i = 0; i < ncol; i++) {
@@ -100,10 +157,38 @@ static int ceil_log2(u64 x) {
     return l;
 }
 
-size_t workspace_bytes(int log_n, bool square, u64 out_len) {
+// Workspace layout:
+//   res                  3 (square) or 6 residue arrays of N words
+//   fmap                 chunks * kCarryChunk limb maps (bytes)
+//   chunk_map, chunk_cin chunks words each
+struct WsLayout {
+    u64* res;
+    u8* fmap;
+    u32* chunk_map;
+    u32* chunk_cin;
+    size_t bytes;
+};
+
+static WsLayout ws_layout(void* ws, int log_n, bool square, u64 out_len) {
     const u64 n = 1ull << log_n;
     const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
-    return (square ? 3 : 6) * n * sizeof(u64) + (1 + chunks) * sizeof(u32) + 256;
+    char* b = (char*)ws;
+    WsLayout L;
+    size_t off = 0;
+    L.res = (u64*)(b + off);
+    off += (square ? 3 : 6) * n * sizeof(u64);
+    L.fmap = (u8*)(b + off);
+    off += chunks * kCarryChunk;
+    L.chunk_map = (u32*)(b + off);
+    off += chunks * sizeof(u32);
+    L.chunk_cin = (u32*)(b + off);
+    off += chunks * sizeof(u32);
+    L.bytes = off;
+    return L;
+}
+
+size_t workspace_bytes(int log_n, bool square, u64 out_len) {
+    return ws_layout(nullptr, log_n, square, out_len).bytes;
 }
 
 // ---------------------------------------------------------------- timing
@@ -364,6 +449,7 @@ struct ColArgs {
     u64* res[6];  // [operand * 3 + prime]
     const u64* tw[3];
     int log_n, s0, s1;
+    int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
 };
 
@@ -372,7 +458,7 @@ __device__ void col_body(const ColArgs& A, int op) {
     extern __shared__ u64 lds[];
     const int logL = A.s1 - A.s0;
     const int logB = A.log_n - A.s1;
-    const int logC = kColLogC;
+    const int logC = A.logC;
     const int C = 1 << logC;
     const int n = C << logL;
     const u64 lam0 = (u64)blockIdx.x << logC;
@@ -405,7 +491,7 @@ __device__ void col_body(const ColArgs& A, int op) {
 }
 
 template <bool FWD>
-__global__ __launch_bounds__(kThreads) void k_col(ColArgs A) {
+__global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const int op = blockIdx.z;
     switch (blockIdx.y) {
         case 0: col_body<0, FWD>(A, op); break;
@@ -426,6 +512,7 @@ struct RowArgs {
     const u64* itw[3];
     u64 r2[3];  // 2^128 mod p (spectrum export to Montgomery form)
     int log_n, s0;
+    int logC;  // 2^logC lines per tile
     int from_limbs;
     int mode;
 };
@@ -434,7 +521,7 @@ template <int P>
 __device__ void row_body(const RowArgs& A) {
     extern __shared__ u64 lds[];
     const int logL = A.log_n - A.s0;
-    const int logC = min(A.s0, kRowTileLog - logL);
+    const int logC = A.logC;
     const int n = 1 << (logL + logC);
     const u64 base = (u64)blockIdx.x << (logL + logC);
     const bool two = (A.mode == kRowMul);
@@ -471,7 +558,7 @@ __device__ void row_body(const RowArgs& A) {
     for (int e = threadIdx.x; e < n; e += blockDim.x) rA[base + e] = tA[e];
 }
 
-__global__ __launch_bounds__(kThreads) void k_row(RowArgs A) {
+__global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
     switch (blockIdx.y) {
         case 0: row_body<0>(A); break;
         case 1: row_body<1>(A); break;
@@ -487,18 +574,26 @@ __global__ __launch_bounds__(kThreads) void k_row(RowArgs A) {
 // s_k = c_k[0] + c_{k-1}[1] + c_{k-2}[2] < 3 * 2^64 and carry_k <= 2
 // (c_k < 2^184).  The carry out of limb k is a map f_k: {0..3} -> {0..3},
 // f_k(c) = hi(s_k) + [lo(s_k) + c >= 2^64]; maps compose associatively, so
-// the serial loop of src/fast_mul.c:378-389 becomes a scan over maps with a
-// single-pass decoupled look-back between chunks.  A map is packed as four
-// 2-bit values, m(c) = (m >> 2c) & 3.
+// the serial loop of src/fast_mul.c:378-389 becomes a scan over maps.  A map
+// is packed as four 2-bit values, m(c) = (m >> 2c) & 3.
+//
+// Three kernels, no inter-workgroup waiting and no contended atomics:
+//   k_crt_local   one chunk of limbs per workgroup: CRT, s_k, writes
+//                 lo(s_k) into out[], the limb maps into fmap[] and the chunk's
+//                 composed map into chunk_map[]
+//   k_chunk_scan  one workgroup: the carry into every chunk, chunk_cin[]
+//   k_carry_apply rescans the chunk's limb maps from its carry-in and adds
+//                 the carries into out[].
 
 struct CarryArgs {
     const u64* res[3];
     u64 conv_len, out_len;
     u64* out;
-    u32* flags;  // [0] chunk counter, [1 + t] status of chunk t
-    u64 K[3];
-    u64 M[3][2];  // P / p_i as two words
-    u64 Pw[3];    // P = p0 p1 p2 as three words
+    u8* fmap;        // one map per limb, kCarryChunk per chunk
+    u32* chunk_map;  // composed map of each chunk
+    u32* chunk_cin;  // carry into each chunk
+    u32 chunks;
+    CrtConsts g;
 };
 
 __device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
@@ -513,103 +608,43 @@ __device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
 __device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (2 * c)) & 3u; }
 constexpr u32 kMapId = 0xE4u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
 
-__device__ __forceinline__ void crt_coef(const CarryArgs& A, u64 k, u64& w0, u64& w1, u64& w2) {
-    w0 = w1 = w2 = 0;
-    if (k >= A.conv_len) return;
-    const u64 x0 = canon<0>(redc<0>(A.res[0][k], A.K[0]));
-    const u64 x1 = canon<1>(redc<1>(A.res[1][k], A.K[1]));
-    const u64 x2 = canon<2>(redc<2>(A.res[2][k], A.K[2]));
-    const u64 xs[3] = {x0, x1, x2};
-    u128 acc_lo = 0;  // bits 0..127
-    u64 acc_hi = 0;   // bits 128..191
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const u128 lo = (u128)xs[i] * A.M[i][0];
-        const u128 hi = (u128)xs[i] * A.M[i][1];
-        // term = lo + hi * 2^64
-        const u128 mid = (lo >> 64) + (u128)(u64)hi;
-        const u128 low128 = (u128)(u64)lo | (mid << 64);
-        const u64 top = (u64)(hi >> 64) + (u64)(mid >> 64);
-        const u128 s = acc_lo + low128;
-        acc_hi += top + (s < acc_lo ? 1ull : 0ull);
-        acc_lo = s;
-    }
-    w0 = (u64)acc_lo;
-    w1 = (u64)(acc_lo >> 64);
-    w2 = acc_hi;
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-        // if (w >= P) w -= P
-        bool ge;
-        if (w2 != A.Pw[2])
-            ge = w2 > A.Pw[2];
-        else if (w1 != A.Pw[1])
-            ge = w1 > A.Pw[1];
-        else
-            ge = w0 >= A.Pw[0];
-        if (ge) {
-            const u64 b0 = w0 < A.Pw[0] ? 1ull : 0ull;
-            w0 -= A.Pw[0];
-            const u128 d1 = (u128)w1 - A.Pw[1] - b0;
-            w1 = (u64)d1;
-            const u64 b1 = (u64)(d1 >> 64) & 1ull;
-            w2 = w2 - A.Pw[2] - b1;
-        }
+// Residues of coefficient k (zero beyond the convolution).
+__device__ __forceinline__ void crt_load(const CarryArgs& A, u64 k, u64& v0, u64& v1, u64& v2) {
+    v0 = v1 = v2 = 0;
+    if (k < A.conv_len) {
+        v0 = A.res[0][k];
+        v1 = A.res[1][k];
+        v2 = A.res[2][k];
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_crt_carry(CarryArgs A) {
-    constexpr int E = kCarryPerThread;
-    constexpr int CH = kCarryChunk;
-    __shared__ u64 s_w1[CH + 2];
-    __shared__ u64 s_w2[CH + 2];
-    __shared__ u32 s_wave[kThreads / 64];
-    __shared__ u32 s_tile;
-    __shared__ u32 s_cin;
-    const int tid = threadIdx.x;
-    if (tid == 0) s_tile = atomicAdd(&A.flags[0], 1u);
-    __syncthreads();
-    const u64 tile = s_tile;
-    const u64 k0 = tile * (u64)CH;
+// Coefficient from its residues as three words (a zero coefficient has zero
+// residues and gives zero).
+__device__ __forceinline__ void crt_coef(const CarryArgs& A, u64 v0, u64 v1, u64 v2, u64& w0, u64& w1,
+                                         u64& w2) {
+    constexpr u64 p0 = Prime<0>::p, p1 = Prime<1>::p, p2 = Prime<2>::p;
+    const CrtConsts& g = A.g;
+    const u64 x0 = canon<0>(redc<0>(v0, g.k00));
+    const u64 y1 = canon<1>(norm2p<1>(redc<1>(v1, g.k11) + 2 * p1 - redc<1>(x0, g.k10)));
+    u64 t = norm2p<2>(redc<2>(v2, g.k22) + 2 * p2 - redc<2>(x0, g.k20));
+    const u64 y2 = canon<2>(norm2p<2>(t + 2 * p2 - redc<2>(y1, g.k21)));
+    // C = x0 + p0 y1 + (p0 p1) y2
+    u64 a_lo, a_hi, b_lo, b_hi, c_lo, c_hi;
+    mul_64x64(p0, y1, a_lo, a_hi);
+    mul_64x64(g.m01[0], y2, b_lo, b_hi);
+    mul_64x64(g.m01[1], y2, c_lo, c_hi);
+    const u128 s0 = (u128)x0 + a_lo + b_lo;
+    const u128 s1 = (u128)(u64)(s0 >> 64) + a_hi + b_hi + c_lo;
+    w0 = (u64)s0;
+    w1 = (u64)s1;
+    w2 = c_hi + (u64)(s1 >> 64);
+}
 
-    u64 lo0[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const u64 k = k0 + (u64)(tid * E + e);
-        u64 w0, w1, w2;
-        crt_coef(A, k, w0, w1, w2);
-        lo0[e] = w0;
-        s_w1[tid * E + e + 2] = w1;
-        s_w2[tid * E + e + 2] = w2;
-    }
-    if (tid < 2) {
-        u64 w0 = 0, w1 = 0, w2 = 0;
-        if (k0 >= (u64)(2 - tid)) crt_coef(A, k0 - 2 + tid, w0, w1, w2);
-        s_w1[tid] = w1;
-        s_w2[tid] = w2;
-    }
-    __syncthreads();
-
-    u64 lo[E];
-    u32 f[E];
-    u32 F = kMapId;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int i = tid * E + e + 2;
-        const u128 s = (u128)lo0[e] + s_w1[i - 1] + s_w2[i - 2];
-        const u64 l = (u64)s;
-        const u32 h = (u32)(s >> 64);
-        lo[e] = l;
-        const u32 f0 = h;
-        const u32 f1 = h + (l == ~0ull ? 1u : 0u);
-        const u32 f2 = h + (l >= ~0ull - 1 ? 1u : 0u);
-        const u32 f3 = h + (l >= ~0ull - 2 ? 1u : 0u);
-        f[e] = f0 | (f1 << 2) | (f2 << 4) | (f3 << 6);
-        F = map_compose(f[e], F);
-    }
-
-    // Block-wide exclusive scan of the per-thread maps (order matters).
-    const int lane = tid & 63, wave = tid >> 6;
+// Exclusive scan of one map per thread over the workgroup (order matters);
+// returns the composition of the maps of lower threads, *agg gets all of them.
+__device__ __forceinline__ u32 block_excl_scan_maps(u32 F, u32* s_wave, u32* agg) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nw = blockDim.x >> 6;
     u32 incl = F;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -622,43 +657,128 @@ __global__ __launch_bounds__(kThreads) void k_crt_carry(CarryArgs A) {
     __syncthreads();
     u32 wpre = kMapId;
     for (int w = 0; w < wave; w++) wpre = map_compose(s_wave[w], wpre);
-    excl = map_compose(excl, wpre);
+    u32 all = kMapId;
+    for (int w = 0; w < nw; w++) all = map_compose(s_wave[w], all);
+    *agg = all;
+    return map_compose(excl, wpre);
+}
 
-    if (tid == 0) {
-        u32 agg = kMapId;
-        for (int w = 0; w < kThreads / 64; w++) agg = map_compose(s_wave[w], agg);
-        u32* fl = A.flags + 1;
-        u32 cin = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&fl[0], (2u << 8) | map_apply(agg, 0), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&fl[tile], (1u << 8) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            u32 acc = kMapId;
-            long long j = (long long)tile - 1;
-            while (true) {
-                const u32 v = __hip_atomic_load(&fl[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const u32 st = v >> 8;
-                if (st == 0) continue;
-                if (st == 2) {
-                    cin = map_apply(acc, v & 3u);
-                    break;
-                }
-                acc = map_compose(acc, v & 0xFFu);
-                j--;
-            }
-            __hip_atomic_store(&fl[tile], (2u << 8) | map_apply(agg, cin), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_cin = cin;
-    }
-    __syncthreads();
-    u32 c = map_apply(excl, s_cin);
+// Coefficients are computed with thread-strided (coalesced) indices into LDS;
+// the limb scan then reads LDS with kCarryPerThread consecutive limbs per
+// thread, and results go back to HBM through LDS with coalesced stores.
+static_assert(kCarryPerThread == 4, "limb maps of one thread are packed in one u32");
+
+__global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
+    constexpr int E = kCarryPerThread;
+    constexpr int CH = kCarryChunk;
+    __shared__ u64 s_w0[CH];
+    __shared__ u64 s_w1[CH + 2];
+    __shared__ u64 s_w2[CH + 2];
+    __shared__ u32 s_wave[kThreads / 64];
+    const int tid = threadIdx.x;
+    const u64 tile = blockIdx.x;
+    const u64 k0 = tile * (u64)CH;
+
+    u64 v[E][3];
+#pragma unroll
+    for (int e = 0; e < E; e++) crt_load(A, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const u64 k = k0 + (u64)(tid * E + e);
-        if (k < A.out_len) A.out[k] = lo[e] + c;
-        c = map_apply(f[e], c);
+        const int i = e * kThreads + tid;
+        u64 w0, w1, w2;
+        crt_coef(A, v[e][0], v[e][1], v[e][2], w0, w1, w2);
+        s_w0[i] = w0;
+        s_w1[i + 2] = w1;
+        s_w2[i + 2] = w2;
+    }
+    if (tid < 2) {
+        u64 v0 = 0, v1 = 0, v2 = 0, w0, w1, w2;
+        if (k0 >= (u64)(2 - tid)) crt_load(A, k0 - 2 + tid, v0, v1, v2);
+        crt_coef(A, v0, v1, v2, w0, w1, w2);
+        s_w1[tid] = w1;
+        s_w2[tid] = w2;
+    }
+    __syncthreads();
+
+    u32 F = kMapId;
+    u64 packed = 0;
+    u64 lo[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = tid * E + e;
+        const u128 s = (u128)s_w0[i] + s_w1[i + 1] + s_w2[i];
+        const u64 l = (u64)s;
+        const u32 h = (u32)(s >> 64);
+        const u32 f0 = h;
+        const u32 f1 = h + (l == ~0ull ? 1u : 0u);
+        const u32 f2 = h + (l >= ~0ull - 1 ? 1u : 0u);
+        const u32 f3 = h + (l >= ~0ull - 2 ? 1u : 0u);
+        const u32 f = f0 | (f1 << 2) | (f2 << 4) | (f3 << 6);
+        packed |= (u64)f << (8 * e);
+        F = map_compose(f, F);
+        lo[e] = l;
+    }
+    *(u32*)(A.fmap + k0 + (u64)tid * E) = (u32)packed;
+    __syncthreads();  // all reads of s_w0 done
+#pragma unroll
+    for (int e = 0; e < E; e++) s_w0[tid * E + e] = lo[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = e * kThreads + tid;
+        if (k0 + (u64)i < A.out_len) A.out[k0 + i] = s_w0[i];
+    }
+
+    u32 agg;
+    (void)block_excl_scan_maps(F, s_wave, &agg);
+    if (tid == 0) A.chunk_map[tile] = agg;
+}
+
+// One workgroup: carry into every chunk from the chunk maps.  Thread t owns a
+// contiguous segment of chunks.
+__global__ __launch_bounds__(1024) void k_chunk_scan(CarryArgs A) {
+    __shared__ u32 s_wave[1024 / 64];
+    const int tid = threadIdx.x;
+    const u32 n = A.chunks;
+    const u32 per = (n + blockDim.x - 1) / blockDim.x;
+    const u32 beg = min(n, (u32)tid * per), end = min(n, beg + per);
+    u32 seg = kMapId;
+    for (u32 i = beg; i < end; i++) seg = map_compose(A.chunk_map[i], seg);
+    u32 all;
+    const u32 excl = block_excl_scan_maps(seg, s_wave, &all);
+    u32 c = map_apply(excl, 0);
+    for (u32 i = beg; i < end; i++) {
+        A.chunk_cin[i] = c;
+        c = map_apply(A.chunk_map[i], c);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
+    constexpr int E = kCarryPerThread;
+    constexpr int CH = kCarryChunk;
+    __shared__ u32 s_wave[kThreads / 64];
+    __shared__ u8 s_c[CH];
+    const int tid = threadIdx.x;
+    const u64 tile = blockIdx.x;
+    const u64 k0 = tile * (u64)CH;
+    const u32 packed = *(const u32*)(A.fmap + k0 + (u64)tid * E);
+    u32 F = kMapId;
+#pragma unroll
+    for (int e = 0; e < E; e++) F = map_compose((u32)(packed >> (8 * e)) & 0xFFu, F);
+    u32 agg;
+    const u32 excl = block_excl_scan_maps(F, s_wave, &agg);
+    u32 c = map_apply(excl, A.chunk_cin[tile]);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        s_c[tid * E + e] = (u8)c;
+        c = map_apply((u32)(packed >> (8 * e)) & 0xFFu, c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int i = e * kThreads + tid;
+        const u32 ci = s_c[i];
+        if (ci != 0 && k0 + (u64)i < A.out_len) A.out[k0 + i] += ci;
     }
 }
 
@@ -766,21 +886,18 @@ hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
 
 // ------------------------------------------------------------- pipelines
 
-static void fill_carry_consts(CarryArgs& C, int log_n) {
-    const CrtConsts k = crt_consts(log_n);
-    for (int i = 0; i < 3; i++) {
-        C.K[i] = k.k[i];
-        const u128 m = (u128)kP[(i + 1) % 3] * kP[(i + 2) % 3];
-        C.M[i][0] = (u64)m;
-        C.M[i][1] = (u64)(m >> 64);
-    }
-    const u128 m01 = (u128)kP[0] * kP[1];
-    const u128 lo = (u128)(u64)m01 * kP[2];
-    const u128 hi = (u128)(u64)(m01 >> 64) * kP[2];
-    C.Pw[0] = (u64)lo;
-    const u128 mid = (lo >> 64) + (u64)hi;
-    C.Pw[1] = (u64)mid;
-    C.Pw[2] = (u64)(hi >> 64) + (u64)(mid >> 64);
+static void fill_carry_consts(CarryArgs& C, int log_n) { C.g = crt_consts(log_n); }
+
+// Tiles above 64 KiB of LDS need the per-kernel limit raised (gfx950 has
+// 160 KiB of LDS per workgroup).
+static void configure_kernels() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const int lds = 160 * 1024;
+        (void)hipFuncSetAttribute((const void*)k_col<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        (void)hipFuncSetAttribute((const void*)k_col<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        (void)hipFuncSetAttribute((const void*)k_row, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    });
 }
 
 static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
@@ -790,13 +907,12 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
     const int log_n = ceil_log2(conv_len);
     if (log_n > kMaxLogN || log_n > tw.log_t) return hipErrorInvalidValue;
     const u64 N = 1ull << log_n;
+    const Tuning& tu = tuning();
+    configure_kernels();
     const Plan pl = make_plan(log_n);
-    u64* res = (u64*)ws;
-    u32* flags = (u32*)(res + (square ? 3 : 6) * N);
+    const WsLayout W = ws_layout(ws, log_n, square, out_len);
+    u64* res = W.res;
     const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
-
-    FMH_LAUNCH("fill_flags", k_fill_u32, dim3((unsigned)((chunks + 1 + 255) / 256)), dim3(256), 0, s,
-               flags, chunks + 1, 0u);
 
     u64* rp[6];
     for (int i = 0; i < 6; i++) rp[i] = res + (u64)i * N;
@@ -815,9 +931,10 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
         const int logL = C.s1 - C.s0;
-        const u64 tiles = N >> (logL + kColLogC);
-        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, nops), dim3(kThreads), lds, s, C);
+        C.logC = tu.col_logc;
+        const u64 tiles = N >> (logL + C.logC);
+        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, nops), dim3(tu.col_threads), lds, s, C);
     }
     {
         RowArgs R{};
@@ -835,10 +952,11 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
         R.from_limbs = (pl.n_col == 0);
         R.mode = square ? kRowSqr : kRowMul;
         const int logL = log_n - pl.row_s;
-        const int logC = std::min(pl.row_s, kRowTileLog - logL);
+        const int logC = std::min(pl.row_s, tu.row_tile - logL);
+        R.logC = logC;
         const u64 tiles = N >> (logL + logC);
         const size_t lds = (size_t)(square ? 1 : 2) * ((size_t)1 << (logL + logC)) * sizeof(u64);
-        FMH_LAUNCH(square ? "row_sqr" : "row_mul", k_row, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds,
+        FMH_LAUNCH(square ? "row_sqr" : "row_mul", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds,
                    s, R);
     }
     for (int pi = pl.n_col - 1; pi >= 0; pi--) {
@@ -850,9 +968,10 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = 0;
         const int logL = C.s1 - C.s0;
-        const u64 tiles = N >> (logL + kColLogC);
-        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
-        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, C);
+        C.logC = tu.col_logc;
+        const u64 tiles = N >> (logL + C.logC);
+        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, 1), dim3(tu.col_threads), lds, s, C);
     }
     {
         CarryArgs K{};
@@ -860,9 +979,14 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
         K.conv_len = conv_len;
         K.out_len = out_len;
         K.out = out;
-        K.flags = flags;
+        K.fmap = W.fmap;
+        K.chunk_map = W.chunk_map;
+        K.chunk_cin = W.chunk_cin;
+        K.chunks = (u32)chunks;
         fill_carry_consts(K, log_n);
-        FMH_LAUNCH("crt_carry", k_crt_carry, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
+        FMH_LAUNCH("crt_local", k_crt_local, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
+        FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3(1), dim3(1024), 0, s, K);
+        FMH_LAUNCH("carry_apply", k_carry_apply, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
     }
     return hipSuccess;
 }
@@ -899,8 +1023,10 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
                             const Twiddles& tw, hipStream_t s) {
     if (log_n > kMaxLogN || log_n > tw.log_t || prime < 0 || prime > 2) return hipErrorInvalidValue;
     const u64 N = 1ull << log_n;
+    const Tuning& tu = tuning();
+    configure_kernels();
     const Plan pl = make_plan(log_n);
-    u64* res = (u64*)ws;
+    u64* res = ws_layout(ws, log_n, true, 0).res;
     u64* rp[6];
     for (int i = 0; i < 3; i++) rp[i] = rp[3 + i] = res + (u64)i * N;
     for (int pi = 0; pi < pl.n_col; pi++) {
@@ -914,9 +1040,10 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
         const int logL = C.s1 - C.s0;
-        const u64 tiles = N >> (logL + kColLogC);
-        const size_t lds = ((size_t)1 << (logL + kColLogC)) * sizeof(u64);
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, C);
+        C.logC = tu.col_logc;
+        const u64 tiles = N >> (logL + C.logC);
+        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, 1), dim3(tu.col_threads), lds, s, C);
     }
     RowArgs R{};
     R.limbs[0] = R.limbs[1] = a;
@@ -934,10 +1061,11 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
     R.from_limbs = (pl.n_col == 0);
     R.mode = kRowFwdOnly;
     const int logL = log_n - pl.row_s;
-    const int logC = std::min(pl.row_s, kRowTileLog - logL);
+    const int logC = std::min(pl.row_s, tu.row_tile - logL);
+    R.logC = logC;
     const u64 tiles = N >> (logL + logC);
     const size_t lds = ((size_t)1 << (logL + logC)) * sizeof(u64);
-    FMH_LAUNCH("row_fwd", k_row, dim3((unsigned)tiles, 3, 1), dim3(kThreads), lds, s, R);
+    FMH_LAUNCH("row_fwd", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
     return hipMemcpyAsync(out, rp[prime], N * sizeof(u64), hipMemcpyDeviceToDevice, s);
 }
 
