@@ -33,6 +33,9 @@ __all__ = [
     "sqr_device",
     "forward_spectrum",
     "copy_device",
+    "FmhProduct",
+    "mul_batch_device",
+    "Batch",
     "timing_enable",
     "timing_collect",
     "FmhError",
@@ -50,6 +53,12 @@ _LIB = None
 
 class FmhError(RuntimeError):
     pass
+
+
+class FmhProduct(ctypes.Structure):
+    """struct fmh_product of include/fmh.h (device pointers)."""
+
+    _fields_ = [("in1", _vp), ("len1", _u64), ("in2", _vp), ("len2", _u64), ("out", _vp)]
 
 
 def declared_symbols(header: str = HEADER_PATH) -> list[str]:
@@ -85,6 +94,10 @@ def lib() -> ctypes.CDLL:
     L.fmh_stream_destroy.argtypes = [_vp]
     L.fmh_stream_synchronize.argtypes = [_vp]
     L.fmh_copy_device.argtypes = [_vp, _vp, _u64, _vp]
+    L.fmh_mul_batch_device.argtypes = [ctypes.POINTER(FmhProduct), _u64, _vp]
+    L.fmh_batch_create.argtypes = [ctypes.POINTER(FmhProduct), _u64, ctypes.POINTER(_vp)]
+    L.fmh_batch_run.argtypes = [_vp, _vp]
+    L.fmh_batch_destroy.argtypes = [_vp]
     L.fmh_last_error.restype = ctypes.c_char_p
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
     L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
@@ -232,6 +245,50 @@ def copy_device(dst: DeviceBuffer, src: DeviceBuffer, n: int, stream: Stream | N
         raise ValueError("device buffers too small")
     _check(lib().fmh_copy_device(dst.ptr, src.ptr, n, stream.handle if stream else None),
            "fmh_copy_device")
+
+
+def _products(items) -> ctypes.Array:
+    """items: (a, na, b, nb, out) with DeviceBuffers (b None for a square)."""
+    arr = (FmhProduct * max(1, len(items)))()
+    for i, (a, na, b, nb, out) in enumerate(items):
+        nb_eff = na if b is None else nb
+        if a.n < na or (b is not None and b.n < nb) or out.n < na + nb_eff:
+            raise ValueError(f"batch item {i}: device buffers too small")
+        arr[i] = FmhProduct(a.ptr, na, None if b is None else b.ptr, nb_eff if b is not None else 0,
+                            out.ptr)
+    return arr
+
+
+def mul_batch_device(items, stream: Stream | None = None) -> None:
+    """Enqueue many products (device buffers) as one batch on `stream`."""
+    arr = _products(items)
+    _check(lib().fmh_mul_batch_device(arr, len(items), stream.handle if stream else None),
+           "fmh_mul_batch_device")
+
+
+class Batch:
+    """A planned batch: workspace and descriptors allocated once, launches
+    captured in a HIP graph; run() replays it."""
+
+    def __init__(self, items):
+        arr = _products(items)
+        h = _vp()
+        _check(lib().fmh_batch_create(arr, len(items), ctypes.byref(h)), "fmh_batch_create")
+        self.handle = h.value
+
+    def run(self, stream: Stream | None = None) -> None:
+        _check(lib().fmh_batch_run(self.handle, stream.handle if stream else None), "fmh_batch_run")
+
+    def close(self) -> None:
+        if self.handle is not None:
+            _check(lib().fmh_batch_destroy(self.handle), "fmh_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def timing_enable(on: bool) -> None:

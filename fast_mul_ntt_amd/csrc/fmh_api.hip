@@ -10,9 +10,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/fmh.h"
 #include "fmh_internal.h"
@@ -95,8 +97,7 @@ int ensure_buffer(Buffer& b, size_t bytes, hipStream_t s) {
 
 int ensure_twiddles(DevCtx& c, int log_n) {
     if (c.tw.log_t >= log_n) return 0;
-    hipError_t e = hipDeviceSynchronize();  // tables may be read by in-flight work
-    if (e != hipSuccess) return set_err("hipDeviceSynchronize", e);
+    hipError_t e;
     int lt = log_n < kMinTwiddleLog ? kMinTwiddleLog : log_n;
     if (lt > kMaxLogN) lt = kMaxLogN;
     e = build_twiddles(c.tw, lt, c.stream);
@@ -104,29 +105,61 @@ int ensure_twiddles(DevCtx& c, int log_n) {
     return 0;
 }
 
-int check_lengths(u64 len1, u64 len2) {
-    if (len1 + len2 - 1 > (1ull << kMaxLogN)) return set_err("product longer than fmh_max_ntt_len()");
+// Pinned host staging of a batch's descriptors on one stream; `done` is
+// recorded after the copy that reads it.
+struct Staging {
+    void* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+};
+std::map<hipStream_t, Staging> g_stage;
+
+// Enqueue a set of products on stream s; callers hold g_mu.
+int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
+    Job job;
+    if (job_build(job, in, count)) return set_err("product longer than fmh_max_ntt_len()");
+    if (ensure_twiddles(c, job.max_log_n)) return -1;
+    Buffer& w = c.ws[s];
+    if (ensure_buffer(w, job_ws_bytes(job), s)) return -1;
+    job_bind(job, w.ptr);
+    hipError_t e;
+    if (job.items.size() > 1) {
+        // descriptors go through pinned memory so the copy can be asynchronous
+        Staging& st = g_stage[s];
+        const size_t nd = job.items.size() * sizeof(ProductDesc);
+        const size_t nc = job.chunk_item.size() * sizeof(u32);
+        if (st.done) {
+            e = hipEventSynchronize(st.done);
+            if (e != hipSuccess) return set_err("hipEventSynchronize", e);
+        } else {
+            e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+            if (e != hipSuccess) return set_err("hipEventCreate", e);
+        }
+        if (st.bytes < nd + nc) {
+            if (st.host) (void)hipHostFree(st.host);
+            st.host = nullptr;
+            st.bytes = 0;
+            e = hipHostMalloc(&st.host, nd + nc + (nd + nc) / 4, hipHostMallocDefault);
+            if (e != hipSuccess) return set_err("hipHostMalloc", e);
+            st.bytes = nd + nc + (nd + nc) / 4;
+        }
+        memcpy(st.host, job.items.data(), nd);
+        memcpy((char*)st.host + nd, job.chunk_item.data(), nc);
+        e = hipMemcpyAsync(job.d_items, st.host, nd, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(job.d_chunk_item, (char*)st.host + nd, nc, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(st.done, s);
+        if (e != hipSuccess) return set_err("descriptor upload", e);
+    }
+    e = job_launch(job, c.tw, s);
+    if (e != hipSuccess) return set_err("kernel launch", e);
     return 0;
 }
 
-// Enqueue a product on stream s; callers hold g_mu.
 int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
                 hipStream_t s) {
-    if (na == 0 || (!square && nb == 0)) {
-        hipError_t e = enqueue_zero(out, square ? 2 * na : na + nb, s);
-        return e == hipSuccess ? 0 : set_err("enqueue_zero", e);
-    }
-    const u64 nb_eff = square ? na : nb;
-    if (check_lengths(na, nb_eff)) return -1;
-    const u64 out_len = na + nb_eff;
-    const int log_n = ceil_log2(out_len - 1);
-    if (ensure_twiddles(c, log_n)) return -1;
-    Buffer& w = c.ws[s];
-    if (ensure_buffer(w, workspace_bytes(log_n, square, out_len), s)) return -1;
-    hipError_t e = square ? enqueue_sqr(a, na, out, w.ptr, c.tw, s)
-                          : enqueue_mul(a, na, b, nb, out, w.ptr, c.tw, s);
-    if (e != hipSuccess) return set_err("kernel launch", e);
-    return 0;
+    const ProductIn in{a, na, square ? nullptr : b, square ? na : nb, out};
+    return run_job(c, &in, 1, s);
 }
 
 [[noreturn]] void die(const char* fn) {
@@ -215,13 +248,99 @@ int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len
     if (ensure_twiddles(*c, log_n)) return -1;
     hipStream_t s = (hipStream_t)stream;
     Buffer& w = c->ws[s];
-    if (ensure_buffer(w, workspace_bytes(log_n, true, 0), s)) return -1;
+    if (ensure_buffer(w, spectrum_ws_bytes(log_n), s)) return -1;
     hipError_t e = enqueue_spectrum(prime, d_in, len, log_n, d_out, w.ptr, c->tw, s);
     if (e != hipSuccess) return set_err("kernel launch", e);
     return 0;
 }
 
 u64 fmh_max_ntt_len(void) { return 1ull << kMaxLogN; }
+
+int fmh_mul_batch_device(const fmh_product* items, u64 count, fmh_stream_t stream) {
+    if (count && !items) return set_err("items is NULL");
+    std::vector<ProductIn> in(count);
+    for (u64 i = 0; i < count; i++)
+        in[i] = ProductIn{items[i].in1, items[i].len1, items[i].in2, items[i].len2, items[i].out};
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    return run_job(*c, in.data(), in.size(), (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+struct fmh_batch {
+    Job job;
+    void* ws = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int fmh_batch_create(const fmh_product* items, u64 count, fmh_batch** out) {
+    if (!out || (count && !items)) return set_err("NULL argument");
+    *out = nullptr;
+    std::vector<ProductIn> in(count);
+    for (u64 i = 0; i < count; i++)
+        in[i] = ProductIn{items[i].in1, items[i].len1, items[i].in2, items[i].len2, items[i].out};
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    fmh_batch* b = new fmh_batch();
+    auto fail = [&](const char* what, hipError_t e) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->ws) (void)hipFree(b->ws);
+        delete b;
+        return e == hipSuccess ? set_err(what) : set_err(what, e);
+    };
+    if (job_build(b->job, in.data(), in.size())) return fail("product longer than fmh_max_ntt_len()", hipSuccess);
+    if (ensure_twiddles(*c, b->job.max_log_n)) {
+        delete b;
+        return -1;
+    }
+    prepare_kernels();
+    hipError_t e = hipMalloc(&b->ws, std::max<size_t>(job_ws_bytes(b->job), 256));
+    if (e != hipSuccess) return fail("hipMalloc", e);
+    job_bind(b->job, b->ws);
+    e = job_upload(b->job, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail("descriptor upload", e);
+    // capture the kernels only (no copies, no allocation, no per-kernel events)
+    const bool timing = timing_enabled();
+    timing_enable(false);
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = job_launch(b->job, c->tw, cs);
+        e = hipStreamEndCapture(cs, &b->graph);
+        if (le != hipSuccess) e = le;
+    }
+    timing_enable(timing);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail("graph capture", e);
+    *out = b;
+    return 0;
+}
+
+int fmh_batch_run(fmh_batch* b, fmh_stream_t stream) {
+    if (!b) return set_err("NULL batch");
+    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : set_err("hipGraphLaunch", e);
+}
+
+int fmh_batch_destroy(fmh_batch* b) {
+    if (!b) return 0;
+    hipError_t e = hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ws) (void)hipFree(b->ws);
+    delete b;
+    return e == hipSuccess ? 0 : set_err("hipDeviceSynchronize", e);
+}
 
 int fmh_copy_device(u64* d_dst, const u64* d_src, u64 n, fmh_stream_t stream) {
     hipError_t e = enqueue_copy(d_dst, d_src, n, (hipStream_t)stream);

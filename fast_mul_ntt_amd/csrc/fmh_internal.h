@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "fmh_field.h"
 
@@ -61,23 +63,67 @@ struct Twiddles {
 
 hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s);
 
-// Workspace layout for one product of transform length N = 2^log_n:
-//   res[0..2]  residues of operand A (and the product), N words each
-//   res[3..5]  residues of operand B (multiply only), N words each
-//   flags      carry look-back: 1 counter + one word per chunk (u32)
-size_t workspace_bytes(int log_n, bool square, u64 out_len);
+// One product as launched: device pointers and its share of the workspace.
+struct ProductDesc {
+    const u64* a;
+    const u64* b;  // equals a for a square
+    u64 na, nb;
+    u64* out;      // na + nb limbs
+    u64* res;      // residues: 3 (square) or 6 arrays of 2^log_n words
+    u32 log_n;
+    u32 square;
+    u32 chunk0;    // first carry chunk of this product in the job's chunk arrays
+    u32 nchunks;
+    CrtConsts crt;
+};
 
-// Enqueue the whole device pipeline on stream s.  All pointers are device
-// pointers; `ws` must hold workspace_bytes(...) bytes.
-hipError_t enqueue_mul(const u64* a, u64 na, const u64* b, u64 nb, u64* out, void* ws,
-                       const Twiddles& tw, hipStream_t s);
-hipError_t enqueue_sqr(const u64* a, u64 na, u64* out, void* ws, const Twiddles& tw,
-                       hipStream_t s);
+// A product as requested: b == nullptr (or b == a with nb == na) is a square.
+struct ProductIn {
+    const u64* a;
+    u64 na;
+    const u64* b;
+    u64 nb;
+    u64* out;
+};
+
+// A set of products launched together: one product, or a batch of products
+// of any sizes.  Products are grouped into size classes (square flag, log_n)
+// and each class runs as one launch per pass; the CRT and carry kernels cover
+// all products at once.
+struct Job {
+    std::vector<ProductDesc> items;  // sorted by (square, log_n)
+    std::vector<u64> res_off;        // residue offset of each item (words)
+    std::vector<u32> chunk_item;     // product index of each carry chunk
+    std::vector<std::pair<u64*, u64>> zeros;  // outputs of products with an empty operand
+    u64 res_words = 0;
+    u32 chunks = 0;
+    int max_log_n = 0;
+    // regions of the workspace, set by job_bind
+    u8* fmap = nullptr;
+    u32* chunk_map = nullptr;
+    u32* chunk_cin = nullptr;
+    ProductDesc* d_items = nullptr;  // device copy of items (batches only)
+    u32* d_chunk_item = nullptr;
+};
+
+// One-time kernel attributes and constant tables (call before graph capture).
+void prepare_kernels();
+
+// Fills the job; returns 0, or -1 if a product is longer than 2^kMaxLogN.
+int job_build(Job& j, const ProductIn* in, size_t count);
+size_t job_ws_bytes(const Job& j);
+void job_bind(Job& j, void* ws);
+// Copies the descriptors of a batch to the device (nothing for one product).
+hipError_t job_upload(const Job& j, hipStream_t s);
+// Enqueues the kernels (no copies, no allocation: capturable in a graph).
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s);
+
 hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s);
 // Plain device copy of n words (HBM calibration kernel for profilers).
 hipError_t enqueue_copy(u64* dst, const u64* src, u64 n, hipStream_t s);
 // Forward spectrum of one operand for one prime, canonical Montgomery form,
-// reference (bit-reversed) order.  ws holds at least 2^log_n words.
+// reference (bit-reversed) order.  ws holds spectrum_ws_bytes(log_n) bytes.
+size_t spectrum_ws_bytes(int log_n);
 hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
                             const Twiddles& tw, hipStream_t s);
 

@@ -157,40 +157,6 @@ static int ceil_log2(u64 x) {
     return l;
 }
 
-// Workspace layout:
-//   res                  3 (square) or 6 residue arrays of N words
-//   fmap                 chunks * kCarryChunk limb maps (bytes)
-//   chunk_map, chunk_cin chunks words each
-struct WsLayout {
-    u64* res;
-    u8* fmap;
-    u32* chunk_map;
-    u32* chunk_cin;
-    size_t bytes;
-};
-
-static WsLayout ws_layout(void* ws, int log_n, bool square, u64 out_len) {
-    const u64 n = 1ull << log_n;
-    const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
-    char* b = (char*)ws;
-    WsLayout L;
-    size_t off = 0;
-    L.res = (u64*)(b + off);
-    off += (square ? 3 : 6) * n * sizeof(u64);
-    L.fmap = (u8*)(b + off);
-    off += chunks * kCarryChunk;
-    L.chunk_map = (u32*)(b + off);
-    off += chunks * sizeof(u32);
-    L.chunk_cin = (u32*)(b + off);
-    off += chunks * sizeof(u32);
-    L.bytes = off;
-    return L;
-}
-
-size_t workspace_bytes(int log_n, bool square, u64 out_len) {
-    return ws_layout(nullptr, log_n, square, out_len).bytes;
-}
-
 // ---------------------------------------------------------------- timing
 
 static std::mutex g_tmu;
@@ -442,20 +408,31 @@ __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC
 }
 
 // ------------------------------------------------------------ column pass
+//
+// Kernels read their products through descriptors: `items` (device array of
+// a batch) or, for a single product, the copy `one` passed by value.
+
+__device__ __forceinline__ const ProductDesc& product(const ProductDesc* items, const ProductDesc& one,
+                                                      u32 i) {
+    return items ? items[i] : one;
+}
 
 struct ColArgs {
-    const u64* limbs[2];
-    u64 len[2];
-    u64* res[6];  // [operand * 3 + prime]
+    const ProductDesc* items;
+    ProductDesc one;
     const u64* tw[3];
     int log_n, s0, s1;
     int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
+    int nops;  // operands transformed per product: 2 (multiply) or 1
 };
 
 template <int P, bool FWD>
-__device__ void col_body(const ColArgs& A, int op) {
+__device__ void col_body(const ColArgs& A) {
     extern __shared__ u64 lds[];
+    const u32 prod = blockIdx.z / (u32)A.nops;
+    const int op = (int)(blockIdx.z - prod * (u32)A.nops);
+    const ProductDesc& D = product(A.items, A.one, prod);
     const int logL = A.s1 - A.s0;
     const int logB = A.log_n - A.s1;
     const int logC = A.logC;
@@ -465,9 +442,9 @@ __device__ void col_body(const ColArgs& A, int op) {
     const u64 a = lam0 >> logB;
     const u32 b0 = (u32)(lam0 & ((1ull << logB) - 1));
     const u64 base = (a << (logL + logB)) + b0;
-    u64* __restrict__ res = A.res[op * 3 + P];
-    const u64* __restrict__ limbs = A.limbs[op];
-    const u64 len = A.len[op];
+    u64* __restrict__ res = D.res + ((u64)(op * 3 + P) << A.log_n);
+    const u64* __restrict__ limbs = op ? D.b : D.a;
+    const u64 len = op ? D.nb : D.na;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const int j = e >> logC, c = e & (C - 1);
         const u64 g = base + ((u64)j << logB) + (u64)c;
@@ -492,22 +469,28 @@ __device__ void col_body(const ColArgs& A, int op) {
 
 template <bool FWD>
 __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
-    const int op = blockIdx.z;
     switch (blockIdx.y) {
-        case 0: col_body<0, FWD>(A, op); break;
-        case 1: col_body<1, FWD>(A, op); break;
-        default: col_body<2, FWD>(A, op); break;
+        case 0: col_body<0, FWD>(A); break;
+        case 1: col_body<1, FWD>(A); break;
+        default: col_body<2, FWD>(A); break;
     }
 }
 
 // --------------------------------------------------------------- row pass
+//
+// A row tile holds C lines of L = 2^logL contiguous words.  Line lambda of a
+// launch is line (lambda mod 2^s0) of product (lambda >> s0): for a large
+// product the lines of a tile are neighbouring blocks of the same transform,
+// for small products (s0 = 0) every line is a whole product, so one
+// workgroup carries C small transforms at once.
 
 enum RowMode { kRowMul = 0, kRowSqr = 1, kRowFwdOnly = 2 };
+constexpr int kMaxRowLines = 64;
 
 struct RowArgs {
-    const u64* limbs[2];
-    u64 len[2];
-    u64* res[6];
+    const ProductDesc* items;
+    ProductDesc one;
+    u32 count;  // products in the launch
     const u64* tw[3];
     const u64* itw[3];
     u64 r2[3];  // 2^128 mod p (spectrum export to Montgomery form)
@@ -517,27 +500,59 @@ struct RowArgs {
     int mode;
 };
 
+struct RowLine {
+    const u64* a;
+    const u64* b;
+    u64 na, nb;
+    u64* ra;
+    const u64* rb;
+    u64 g0;  // offset of the line inside its product's arrays
+    u32 active;
+};
+
 template <int P>
-__device__ void row_body(const RowArgs& A) {
+__device__ void row_body(const RowArgs& A, RowLine* s_line) {
     extern __shared__ u64 lds[];
     const int logL = A.log_n - A.s0;
     const int logC = A.logC;
-    const int n = 1 << (logL + logC);
-    const u64 base = (u64)blockIdx.x << (logL + logC);
+    const int C = 1 << logC;
+    const int n = C << logL;
     const bool two = (A.mode == kRowMul);
     u64* __restrict__ tA = lds;
     u64* __restrict__ tB = lds + n;
-    u64* __restrict__ rA = A.res[P];
-    const u64* __restrict__ rB = A.res[3 + P];
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const u64 g = base + (u64)e;
-        if (A.from_limbs) {
-            tA[e] = g < A.len[0] ? reduce_word<P>(A.limbs[0][g]) : 0ull;
-            if (two) tB[e] = g < A.len[1] ? reduce_word<P>(A.limbs[1][g]) : 0ull;
-        } else {
-            tA[e] = rA[g];
-            if (two) tB[e] = rB[g];
+    if ((int)threadIdx.x < C) {
+        const u64 lambda = ((u64)blockIdx.x << logC) + threadIdx.x;
+        const u64 prod = lambda >> A.s0;
+        RowLine li{};
+        li.active = prod < A.count ? 1u : 0u;
+        if (li.active) {
+            const ProductDesc& D = product(A.items, A.one, (u32)prod);
+            li.a = D.a;
+            li.b = D.b;
+            li.na = D.na;
+            li.nb = D.nb;
+            li.ra = D.res + ((u64)P << A.log_n);
+            li.rb = D.res + ((u64)(3 + P) << A.log_n);
+            li.g0 = (lambda & ((1ull << A.s0) - 1)) << logL;
         }
+        s_line[threadIdx.x] = li;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const RowLine& li = s_line[e >> logL];
+        const u64 g = li.g0 + (u64)(e & ((1 << logL) - 1));
+        u64 va = 0, vb = 0;
+        if (li.active) {
+            if (A.from_limbs) {
+                va = g < li.na ? reduce_word<P>(li.a[g]) : 0ull;
+                if (two) vb = g < li.nb ? reduce_word<P>(li.b[g]) : 0ull;
+            } else {
+                va = li.ra[g];
+                if (two) vb = li.rb[g];
+            }
+        }
+        tA[e] = va;
+        if (two) tB[e] = vb;
     }
     __syncthreads();
     if (two)
@@ -546,7 +561,10 @@ __device__ void row_body(const RowArgs& A) {
         tile_dif<P, 1, 1>(tA, nullptr, logL, logC, 0, 0u, A.tw[P]);
     if (A.mode == kRowFwdOnly) {
         const u64 r2 = A.r2[P];
-        for (int e = threadIdx.x; e < n; e += blockDim.x) rA[base + e] = canon<P>(redc<P>(tA[e], r2));
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const RowLine& li = s_line[e >> logL];
+            if (li.active) li.ra[li.g0 + (e & ((1 << logL) - 1))] = canon<P>(redc<P>(tA[e], r2));
+        }
         return;
     }
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -555,14 +573,18 @@ __device__ void row_body(const RowArgs& A) {
     }
     __syncthreads();
     tile_dit<P, 1>(tA, logL, logC, 0, 0u, A.itw[P]);
-    for (int e = threadIdx.x; e < n; e += blockDim.x) rA[base + e] = tA[e];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const RowLine& li = s_line[e >> logL];
+        if (li.active) li.ra[li.g0 + (e & ((1 << logL) - 1))] = tA[e];
+    }
 }
 
 __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
+    __shared__ RowLine s_line[kMaxRowLines];
     switch (blockIdx.y) {
-        case 0: row_body<0>(A); break;
-        case 1: row_body<1>(A); break;
-        default: row_body<2>(A); break;
+        case 0: row_body<0>(A, s_line); break;
+        case 1: row_body<1>(A, s_line); break;
+        default: row_body<2>(A, s_line); break;
     }
 }
 
@@ -586,14 +608,12 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
 //                 the carries into out[].
 
 struct CarryArgs {
-    const u64* res[3];
-    u64 conv_len, out_len;
-    u64* out;
-    u8* fmap;        // one map per limb, kCarryChunk per chunk
-    u32* chunk_map;  // composed map of each chunk
-    u32* chunk_cin;  // carry into each chunk
-    u32 chunks;
-    CrtConsts g;
+    const ProductDesc* items;
+    ProductDesc one;
+    const u32* chunk_item;  // product of each chunk (nullptr: product 0)
+    u8* fmap;               // one map per limb, kCarryChunk per chunk
+    u32* chunk_map;         // composed map of each chunk
+    u32* chunk_cin;         // carry into each chunk
 };
 
 __device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
@@ -609,21 +629,20 @@ __device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (2 * c)) &
 constexpr u32 kMapId = 0xE4u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
 
 // Residues of coefficient k (zero beyond the convolution).
-__device__ __forceinline__ void crt_load(const CarryArgs& A, u64 k, u64& v0, u64& v1, u64& v2) {
+__device__ __forceinline__ void crt_load(const ProductDesc& D, u64 k, u64& v0, u64& v1, u64& v2) {
     v0 = v1 = v2 = 0;
-    if (k < A.conv_len) {
-        v0 = A.res[0][k];
-        v1 = A.res[1][k];
-        v2 = A.res[2][k];
+    if (k < D.na + D.nb - 1) {
+        v0 = D.res[k];
+        v1 = D.res[k + (1ull << D.log_n)];
+        v2 = D.res[k + (2ull << D.log_n)];
     }
 }
 
 // Coefficient from its residues as three words (a zero coefficient has zero
 // residues and gives zero).
-__device__ __forceinline__ void crt_coef(const CarryArgs& A, u64 v0, u64 v1, u64 v2, u64& w0, u64& w1,
+__device__ __forceinline__ void crt_coef(const CrtConsts& g, u64 v0, u64 v1, u64 v2, u64& w0, u64& w1,
                                          u64& w2) {
     constexpr u64 p0 = Prime<0>::p, p1 = Prime<1>::p, p2 = Prime<2>::p;
-    const CrtConsts& g = A.g;
     const u64 x0 = canon<0>(redc<0>(v0, g.k00));
     const u64 y1 = canon<1>(norm2p<1>(redc<1>(v1, g.k11) + 2 * p1 - redc<1>(x0, g.k10)));
     u64 t = norm2p<2>(redc<2>(v2, g.k22) + 2 * p2 - redc<2>(x0, g.k20));
@@ -677,24 +696,26 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     __shared__ u32 s_wave[kThreads / 64];
     const int tid = threadIdx.x;
     const u64 tile = blockIdx.x;
-    const u64 k0 = tile * (u64)CH;
+    const ProductDesc& D = product(A.items, A.one, A.chunk_item ? A.chunk_item[tile] : 0u);
+    const u64 k0 = (tile - D.chunk0) * (u64)CH;
+    const u64 out_len = D.na + D.nb;
 
     u64 v[E][3];
 #pragma unroll
-    for (int e = 0; e < E; e++) crt_load(A, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
+    for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
         u64 w0, w1, w2;
-        crt_coef(A, v[e][0], v[e][1], v[e][2], w0, w1, w2);
+        crt_coef(D.crt, v[e][0], v[e][1], v[e][2], w0, w1, w2);
         s_w0[i] = w0;
         s_w1[i + 2] = w1;
         s_w2[i + 2] = w2;
     }
     if (tid < 2) {
         u64 v0 = 0, v1 = 0, v2 = 0, w0, w1, w2;
-        if (k0 >= (u64)(2 - tid)) crt_load(A, k0 - 2 + tid, v0, v1, v2);
-        crt_coef(A, v0, v1, v2, w0, w1, w2);
+        if (k0 >= (u64)(2 - tid)) crt_load(D, k0 - 2 + tid, v0, v1, v2);
+        crt_coef(D.crt, v0, v1, v2, w0, w1, w2);
         s_w1[tid] = w1;
         s_w2[tid] = w2;
     }
@@ -718,7 +739,7 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         F = map_compose(f, F);
         lo[e] = l;
     }
-    *(u32*)(A.fmap + k0 + (u64)tid * E) = (u32)packed;
+    *(u32*)(A.fmap + tile * CH + (u64)tid * E) = (u32)packed;
     __syncthreads();  // all reads of s_w0 done
 #pragma unroll
     for (int e = 0; e < E; e++) s_w0[tid * E + e] = lo[e];
@@ -726,7 +747,7 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
-        if (k0 + (u64)i < A.out_len) A.out[k0 + i] = s_w0[i];
+        if (k0 + (u64)i < out_len) D.out[k0 + i] = s_w0[i];
     }
 
     u32 agg;
@@ -734,22 +755,25 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     if (tid == 0) A.chunk_map[tile] = agg;
 }
 
-// One workgroup: carry into every chunk from the chunk maps.  Thread t owns a
-// contiguous segment of chunks.
-__global__ __launch_bounds__(1024) void k_chunk_scan(CarryArgs A) {
-    __shared__ u32 s_wave[1024 / 64];
+// One workgroup per product: the carry into each of its chunks.  Thread t
+// owns a contiguous segment of the product's chunks.
+__global__ __launch_bounds__(kThreads) void k_chunk_scan(CarryArgs A) {
+    __shared__ u32 s_wave[kThreads / 64];
     const int tid = threadIdx.x;
-    const u32 n = A.chunks;
+    const ProductDesc& D = product(A.items, A.one, blockIdx.x);
+    const u32 n = D.nchunks;
+    const u32* maps = A.chunk_map + D.chunk0;
+    u32* cin = A.chunk_cin + D.chunk0;
     const u32 per = (n + blockDim.x - 1) / blockDim.x;
     const u32 beg = min(n, (u32)tid * per), end = min(n, beg + per);
     u32 seg = kMapId;
-    for (u32 i = beg; i < end; i++) seg = map_compose(A.chunk_map[i], seg);
+    for (u32 i = beg; i < end; i++) seg = map_compose(maps[i], seg);
     u32 all;
     const u32 excl = block_excl_scan_maps(seg, s_wave, &all);
     u32 c = map_apply(excl, 0);
     for (u32 i = beg; i < end; i++) {
-        A.chunk_cin[i] = c;
-        c = map_apply(A.chunk_map[i], c);
+        cin[i] = c;
+        c = map_apply(maps[i], c);
     }
 }
 
@@ -760,8 +784,10 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     __shared__ u8 s_c[CH];
     const int tid = threadIdx.x;
     const u64 tile = blockIdx.x;
-    const u64 k0 = tile * (u64)CH;
-    const u32 packed = *(const u32*)(A.fmap + k0 + (u64)tid * E);
+    const ProductDesc& D = product(A.items, A.one, A.chunk_item ? A.chunk_item[tile] : 0u);
+    const u64 k0 = (tile - D.chunk0) * (u64)CH;
+    const u64 out_len = D.na + D.nb;
+    const u32 packed = *(const u32*)(A.fmap + tile * CH + (u64)tid * E);
     u32 F = kMapId;
 #pragma unroll
     for (int e = 0; e < E; e++) F = map_compose((u32)(packed >> (8 * e)) & 0xFFu, F);
@@ -778,7 +804,7 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
         const u32 ci = s_c[i];
-        if (ci != 0 && k0 + (u64)i < A.out_len) A.out[k0 + i] += ci;
+        if (ci != 0 && k0 + (u64)i < out_len) D.out[k0 + i] += ci;
     }
 }
 
@@ -845,27 +871,32 @@ __global__ void k_tw_low(TwArgs A) {
     tab[i] = tab[half + t * (half >> lh)];
 }
 
+// Builds tables of 2^log_t words and then swaps them in.  The tables they
+// replace are never freed: work already enqueued, and graphs captured with
+// them, keep valid pointers (tables only grow, so at most half of the
+// memory held is retired).
 hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
     if (log_t < 1) log_t = 1;
     const u64 T = 1ull << log_t;
-    for (int i = 0; i < 3; i++) {
-        if (tw.fwd[i]) (void)hipFree(tw.fwd[i]);
-        if (tw.inv[i]) (void)hipFree(tw.inv[i]);
-        tw.fwd[i] = tw.inv[i] = nullptr;
-    }
-    tw.log_t = -1;
+    u64* fwd[3] = {nullptr, nullptr, nullptr};
+    u64* inv[3] = {nullptr, nullptr, nullptr};
     TwArgs A{};
     A.log_t = log_t;
     for (int i = 0; i < 3; i++) {
-        hipError_t e = hipMalloc(&tw.fwd[i], T * sizeof(u64));
-        if (e != hipSuccess) return e;
-        e = hipMalloc(&tw.inv[i], T * sizeof(u64));
-        if (e != hipSuccess) return e;
+        hipError_t e = hipMalloc(&fwd[i], T * sizeof(u64));
+        if (e == hipSuccess) e = hipMalloc(&inv[i], T * sizeof(u64));
+        if (e != hipSuccess) {
+            for (int j = 0; j <= i; j++) {
+                if (fwd[j]) (void)hipFree(fwd[j]);
+                if (inv[j]) (void)hipFree(inv[j]);
+            }
+            return e;
+        }
         const u64 p = kP[i];
         const u64 root = h_powmod(kG[i], (p - 1) >> log_t, p);
         const u64 iroot = h_powmod(root, p - 2, p);
-        A.tab[i] = tw.fwd[i];
-        A.tab[3 + i] = tw.inv[i];
+        A.tab[i] = fwd[i];
+        A.tab[3 + i] = inv[i];
         A.root[i] = h_mont(root, p);
         A.root[3 + i] = h_mont(iroot, p);
         A.one[i] = h_mont(1, p);
@@ -880,69 +911,174 @@ hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    for (int i = 0; i < 3; i++) {
+        tw.fwd[i] = fwd[i];
+        tw.inv[i] = inv[i];
+    }
     tw.log_t = log_t;
     return hipSuccess;
 }
 
 // ------------------------------------------------------------- pipelines
 
-static void fill_carry_consts(CarryArgs& C, int log_n) { C.g = crt_consts(log_n); }
+// Garner constants for every transform length, computed once.
+static const CrtConsts& crt_table(int log_n) {
+    static CrtConsts tab[kMaxLogN + 1];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int l = 0; l <= kMaxLogN; l++) tab[l] = crt_consts(l);
+    });
+    return tab[log_n];
+}
 
 // Tiles above 64 KiB of LDS need the per-kernel limit raised (gfx950 has
 // 160 KiB of LDS per workgroup).
-static void configure_kernels() {
+// Largest dynamic LDS each pass kernel may be launched with: gfx950 offers
+// 160 KiB per workgroup, static LDS included; kernels default to 64 KiB
+// unless the limit is raised.  Launches that would exceed it are refused
+// with an error instead of being sent.
+static size_t g_max_dyn[3] = {0, 0, 0};  // k_col<true>, k_col<false>, k_row
+
+static size_t raise_lds_limit(const void* f) {
+    constexpr size_t kLdsPerGroup = 160 * 1024, kDefault = 64 * 1024;
+    hipFuncAttributes attr{};
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&attr, f) == hipSuccess) stat = attr.sharedSizeBytes;
+    size_t allowed = kLdsPerGroup - stat;
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)allowed) != hipSuccess)
+        allowed = kDefault > stat ? kDefault - stat : 0;
+    (void)hipGetLastError();  // a refused attribute must not surface at the next launch
+    return allowed;
+}
+
+void prepare_kernels() {
+    (void)crt_table(0);
     static std::once_flag once;
     std::call_once(once, [] {
-        const int lds = 160 * 1024;
-        (void)hipFuncSetAttribute((const void*)k_col<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        (void)hipFuncSetAttribute((const void*)k_col<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        (void)hipFuncSetAttribute((const void*)k_row, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        g_max_dyn[0] = raise_lds_limit((const void*)k_col<true>);
+        g_max_dyn[1] = raise_lds_limit((const void*)k_col<false>);
+        g_max_dyn[2] = raise_lds_limit((const void*)k_row);
     });
 }
 
-static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
-                                  void* ws, const Twiddles& tw, hipStream_t s) {
-    const u64 out_len = na + nb;
-    const u64 conv_len = out_len - 1;
-    const int log_n = ceil_log2(conv_len);
-    if (log_n > kMaxLogN || log_n > tw.log_t) return hipErrorInvalidValue;
-    const u64 N = 1ull << log_n;
+int job_build(Job& j, const ProductIn* in, size_t count) {
+    j = Job();
+    for (size_t i = 0; i < count; i++) {
+        const ProductIn& p = in[i];
+        const bool square = p.b == nullptr || (p.b == p.a && p.nb == p.na);
+        const u64 nb = square ? p.na : p.nb;
+        if (p.na == 0 || nb == 0) {
+            if (p.na + nb) j.zeros.push_back({p.out, p.na + nb});
+            continue;
+        }
+        const u64 out_len = p.na + nb;
+        const int log_n = ceil_log2(out_len - 1);
+        if (log_n > kMaxLogN) return -1;
+        ProductDesc d{};
+        d.a = p.a;
+        d.b = square ? p.a : p.b;
+        d.na = p.na;
+        d.nb = nb;
+        d.out = p.out;
+        d.log_n = (u32)log_n;
+        d.square = square ? 1u : 0u;
+        d.nchunks = (u32)((out_len + kCarryChunk - 1) / kCarryChunk);
+        d.crt = crt_table(log_n);
+        j.items.push_back(d);
+    }
+    std::stable_sort(j.items.begin(), j.items.end(), [](const ProductDesc& x, const ProductDesc& y) {
+        return x.square != y.square ? x.square < y.square : x.log_n < y.log_n;
+    });
+    j.res_off.resize(j.items.size());
+    for (size_t i = 0; i < j.items.size(); i++) {
+        ProductDesc& d = j.items[i];
+        j.res_off[i] = j.res_words;
+        j.res_words += (u64)(d.square ? 3 : 6) << d.log_n;
+        d.chunk0 = j.chunks;
+        j.chunks += d.nchunks;
+        for (u32 c = 0; c < d.nchunks; c++) j.chunk_item.push_back((u32)i);
+        j.max_log_n = std::max(j.max_log_n, (int)d.log_n);
+    }
+    return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t job_ws_bytes(const Job& j) {
+    size_t b = align256(j.res_words * sizeof(u64));
+    b += align256((size_t)j.chunks * kCarryChunk);
+    b += 2 * align256((size_t)j.chunks * sizeof(u32));
+    if (j.items.size() > 1) {
+        b += align256(j.items.size() * sizeof(ProductDesc));
+        b += align256((size_t)j.chunks * sizeof(u32));
+    }
+    return b;
+}
+
+void job_bind(Job& j, void* ws) {
+    char* p = (char*)ws;
+    u64* res = (u64*)p;
+    p += align256(j.res_words * sizeof(u64));
+    for (size_t i = 0; i < j.items.size(); i++) j.items[i].res = res + j.res_off[i];
+    j.fmap = (u8*)p;
+    p += align256((size_t)j.chunks * kCarryChunk);
+    j.chunk_map = (u32*)p;
+    p += align256((size_t)j.chunks * sizeof(u32));
+    j.chunk_cin = (u32*)p;
+    p += align256((size_t)j.chunks * sizeof(u32));
+    if (j.items.size() > 1) {
+        j.d_items = (ProductDesc*)p;
+        p += align256(j.items.size() * sizeof(ProductDesc));
+        j.d_chunk_item = (u32*)p;
+    } else {
+        j.d_items = nullptr;
+        j.d_chunk_item = nullptr;
+    }
+}
+
+hipError_t job_upload(const Job& j, hipStream_t s) {
+    if (j.items.size() <= 1) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(j.d_items, j.items.data(), j.items.size() * sizeof(ProductDesc),
+                                  hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(j.d_chunk_item, j.chunk_item.data(), j.chunk_item.size() * sizeof(u32),
+                          hipMemcpyHostToDevice, s);
+}
+
+// Transforms of one size class: items [first, first + count) share log_n and
+// the square flag.
+static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
     const Tuning& tu = tuning();
-    configure_kernels();
+    const ProductDesc& d0 = j.items[first];
+    const int log_n = (int)d0.log_n;
+    const bool square = d0.square != 0;
+    const u64 N = 1ull << log_n;
     const Plan pl = make_plan(log_n);
-    const WsLayout W = ws_layout(ws, log_n, square, out_len);
-    u64* res = W.res;
-    const u64 chunks = (out_len + kCarryChunk - 1) / kCarryChunk;
-
-    u64* rp[6];
-    for (int i = 0; i < 6; i++) rp[i] = res + (u64)i * N;
-
+    const ProductDesc* items = j.d_items ? j.d_items + first : nullptr;
     const int nops = square ? 1 : 2;
     for (int pi = 0; pi < pl.n_col; pi++) {
         ColArgs C{};
-        C.limbs[0] = a;
-        C.limbs[1] = square ? a : b;
-        C.len[0] = na;
-        C.len[1] = square ? na : nb;
-        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        C.items = items;
+        C.one = d0;
         for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
+        C.nops = nops;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, nops), dim3(tu.col_threads), lds, s, C);
+        if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, count * nops), dim3(tu.col_threads), lds, s,
+                   C);
     }
     {
         RowArgs R{};
-        R.limbs[0] = a;
-        R.limbs[1] = b;
-        R.len[0] = na;
-        R.len[1] = nb;
-        for (int i = 0; i < 6; i++) R.res[i] = rp[i];
+        R.items = items;
+        R.one = d0;
+        R.count = count;
         for (int i = 0; i < 3; i++) {
             R.tw[i] = tw.fwd[i];
             R.itw[i] = tw.inv[i];
@@ -952,52 +1088,67 @@ static hipError_t enqueue_product(const u64* a, u64 na, const u64* b, u64 nb, bo
         R.from_limbs = (pl.n_col == 0);
         R.mode = square ? kRowSqr : kRowMul;
         const int logL = log_n - pl.row_s;
-        const int logC = std::min(pl.row_s, tu.row_tile - logL);
+        const u64 lines = (u64)count << pl.row_s;
+        int logC = std::max(0, std::min(tu.row_tile - logL, 6));  // kMaxRowLines = 64
+        while (logC > 0 && (1ull << (logC - 1)) >= lines) logC--;
         R.logC = logC;
-        const u64 tiles = N >> (logL + logC);
+        const u64 tiles = (lines + (1ull << logC) - 1) >> logC;
         const size_t lds = (size_t)(square ? 1 : 2) * ((size_t)1 << (logL + logC)) * sizeof(u64);
+        if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
         FMH_LAUNCH(square ? "row_sqr" : "row_mul", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds,
                    s, R);
     }
     for (int pi = pl.n_col - 1; pi >= 0; pi--) {
         ColArgs C{};
-        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        C.items = items;
+        C.one = d0;
         for (int i = 0; i < 3; i++) C.tw[i] = tw.inv[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = 0;
+        C.nops = 1;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
-        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, 1), dim3(tu.col_threads), lds, s, C);
-    }
-    {
-        CarryArgs K{};
-        for (int i = 0; i < 3; i++) K.res[i] = rp[i];
-        K.conv_len = conv_len;
-        K.out_len = out_len;
-        K.out = out;
-        K.fmap = W.fmap;
-        K.chunk_map = W.chunk_map;
-        K.chunk_cin = W.chunk_cin;
-        K.chunks = (u32)chunks;
-        fill_carry_consts(K, log_n);
-        FMH_LAUNCH("crt_local", k_crt_local, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
-        FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3(1), dim3(1024), 0, s, K);
-        FMH_LAUNCH("carry_apply", k_carry_apply, dim3((unsigned)chunks), dim3(kThreads), 0, s, K);
+        if (lds > g_max_dyn[1]) return hipErrorInvalidValue;
+        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, count), dim3(tu.col_threads), lds, s, C);
     }
     return hipSuccess;
 }
 
-hipError_t enqueue_mul(const u64* a, u64 na, const u64* b, u64 nb, u64* out, void* ws, const Twiddles& tw,
-                       hipStream_t s) {
-    return enqueue_product(a, na, b, nb, false, out, ws, tw, s);
-}
-
-hipError_t enqueue_sqr(const u64* a, u64 na, u64* out, void* ws, const Twiddles& tw, hipStream_t s) {
-    return enqueue_product(a, na, a, na, true, out, ws, tw, s);
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
+    prepare_kernels();
+    if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
+    for (const auto& z : j.zeros) {
+        hipError_t e = enqueue_zero(z.first, z.second, s);
+        if (e != hipSuccess) return e;
+    }
+    if (j.items.empty()) return hipSuccess;
+    if (j.items.size() > 1 && !j.d_items) return hipErrorInvalidValue;
+    constexpr u32 kMaxClass = 32767;  // grid.z holds products x operands
+    size_t first = 0;
+    while (first < j.items.size()) {
+        size_t last = first;
+        while (last < j.items.size() && last - first < kMaxClass && j.items[last].log_n == j.items[first].log_n &&
+               j.items[last].square == j.items[first].square)
+            last++;
+        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s);
+        if (e != hipSuccess) return e;
+        first = last;
+    }
+    CarryArgs K{};
+    K.items = j.d_items;
+    K.one = j.items[0];
+    K.chunk_item = j.d_chunk_item;
+    K.fmap = j.fmap;
+    K.chunk_map = j.chunk_map;
+    K.chunk_cin = j.chunk_cin;
+    FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.items.size()), dim3(kThreads), 0, s, K);
+    FMH_LAUNCH("carry_apply", k_carry_apply, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    return hipSuccess;
 }
 
 __global__ void k_copy(u64* __restrict__ dst, const u64* __restrict__ src, u64 n) {
@@ -1019,36 +1170,40 @@ hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s) {
     return hipSuccess;
 }
 
+size_t spectrum_ws_bytes(int log_n) { return (size_t)3 << log_n << 3; }
+
 hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
                             const Twiddles& tw, hipStream_t s) {
     if (log_n > kMaxLogN || log_n > tw.log_t || prime < 0 || prime > 2) return hipErrorInvalidValue;
+    prepare_kernels();
     const u64 N = 1ull << log_n;
     const Tuning& tu = tuning();
-    configure_kernels();
     const Plan pl = make_plan(log_n);
-    u64* res = ws_layout(ws, log_n, true, 0).res;
-    u64* rp[6];
-    for (int i = 0; i < 3; i++) rp[i] = rp[3 + i] = res + (u64)i * N;
+    ProductDesc d{};
+    d.a = d.b = a;
+    d.na = d.nb = na;
+    d.res = (u64*)ws;
+    d.log_n = (u32)log_n;
+    d.square = 1;
     for (int pi = 0; pi < pl.n_col; pi++) {
         ColArgs C{};
-        C.limbs[0] = C.limbs[1] = a;
-        C.len[0] = C.len[1] = na;
-        for (int i = 0; i < 6; i++) C.res[i] = rp[i];
+        C.one = d;
         for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
+        C.nops = 1;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
         FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, 1), dim3(tu.col_threads), lds, s, C);
     }
     RowArgs R{};
-    R.limbs[0] = R.limbs[1] = a;
-    R.len[0] = R.len[1] = na;
-    for (int i = 0; i < 6; i++) R.res[i] = rp[i];
+    R.one = d;
+    R.count = 1;
     for (int i = 0; i < 3; i++) {
         R.tw[i] = tw.fwd[i];
         R.itw[i] = tw.inv[i];
@@ -1061,12 +1216,13 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
     R.from_limbs = (pl.n_col == 0);
     R.mode = kRowFwdOnly;
     const int logL = log_n - pl.row_s;
-    const int logC = std::min(pl.row_s, tu.row_tile - logL);
+    const int logC = std::max(0, std::min({pl.row_s, tu.row_tile - logL, 6}));
     R.logC = logC;
     const u64 tiles = N >> (logL + logC);
     const size_t lds = ((size_t)1 << (logL + logC)) * sizeof(u64);
+    if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
     FMH_LAUNCH("row_fwd", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
-    return hipMemcpyAsync(out, rp[prime], N * sizeof(u64), hipMemcpyDeviceToDevice, s);
+    return hipMemcpyAsync(out, d.res + ((u64)prime << log_n), N * sizeof(u64), hipMemcpyDeviceToDevice, s);
 }
 
 }  // namespace fmh

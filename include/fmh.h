@@ -52,6 +52,35 @@ int fmh_sqr_device(const u64* d_in1, u64 len1, u64* d_out, fmh_stream_t stream);
 int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len, u64* d_out,
                                 fmh_stream_t stream);
 
+/* ------------------------------------------------------------- batches
+ * No counterpart in the reference (one product per call there).  Many
+ * products of any sizes -- multiplies and squares mixed -- run as one set of
+ * launches: products are grouped by transform length, small transforms are
+ * packed several to a workgroup, and the CRT/carry kernels cover the whole
+ * batch.  All pointers are device pointers; in2 == NULL (or in2 == in1 with
+ * len2 == len1) is a square.  Outputs must not overlap each other or any
+ * input. */
+typedef struct {
+    const u64* in1;
+    u64 len1;
+    const u64* in2;
+    u64 len2;
+    u64* out; /* len1 + len2 limbs (2 * len1 for a square) */
+} fmh_product;
+
+/* One-shot: enqueue the batch on `stream` (scratch is the stream's workspace). */
+int fmh_mul_batch_device(const fmh_product* items, u64 count, fmh_stream_t stream);
+
+/* Planned batch: the workspace and the device copy of the descriptors are
+ * allocated once and the launch sequence is captured in a HIP graph, so a
+ * batch whose operand and output pointers stay the same is re-run with one
+ * graph launch.  fmh_batch_run may be called any number of times; runs on
+ * one batch must not overlap in time (they share its workspace). */
+typedef struct fmh_batch fmh_batch;
+int fmh_batch_create(const fmh_product* items, u64 count, fmh_batch** batch);
+int fmh_batch_run(fmh_batch* batch, fmh_stream_t stream);
+int fmh_batch_destroy(fmh_batch* batch);
+
 /* Largest supported transform length (products with len1 + len2 - 1 up to
  * this many coefficients). */
 u64 fmh_max_ntt_len(void);

@@ -127,3 +127,76 @@ def test_device_entry_points(fm, orc):
         fm.sqr_device(da, a.size, do, s)
         s.synchronize()
         assert np.array_equal(do.download(2 * a.size), orc.sqr(a))
+
+
+def _log_uniform_sizes(rng, count, lo, hi):
+    import math
+    return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(count)]
+
+
+def test_batch_mixed_sizes_vs_oracle(fm, orc):
+    """One batch of products of uneven sizes (multiplies and squares, some
+    zero-length): every result equals the oracle's, as in single calls."""
+    import random
+    rng = random.Random(4001)
+    n = 120
+    l1 = _log_uniform_sizes(rng, n, 1, 40000)
+    l2 = _log_uniform_sizes(rng, n, 1, 40000)
+    sq = [rng.random() < 0.25 for _ in range(n)]
+    l1[3], l2[3] = 0, 17       # empty operand
+    l1[5], l2[5], sq[5] = 7, 0, False
+    host, items, bufs = [], [], []
+    for i in range(n):
+        a = orc.splitmix(10_000 + 2 * i, l1[i])
+        b = None if sq[i] else orc.splitmix(10_001 + 2 * i, l2[i])
+        out_len = 2 * l1[i] if sq[i] else l1[i] + l2[i]
+        da = fm.DeviceBuffer.from_host(a)
+        db = None if sq[i] else fm.DeviceBuffer.from_host(b)
+        do = fm.DeviceBuffer(max(out_len, 1))
+        bufs += [da, do] + ([db] if db else [])
+        items.append((da, l1[i], db, 0 if sq[i] else l2[i], do))
+        host.append((a, b, out_len))
+    try:
+        with fm.Stream() as s:
+            fm.mul_batch_device(items, s)
+            s.synchronize()
+            for i, (a, b, out_len) in enumerate(host):
+                want = orc.sqr(a) if b is None else orc.mul(a, b)
+                got = items[i][4].download(out_len)
+                assert np.array_equal(got, want), (i, l1[i], l2[i], sq[i])
+    finally:
+        for d in bufs:
+            d.free()
+
+
+def test_batch_graph_replays(fm, orc):
+    """A planned batch replays its captured graph: new operand values in the
+    same buffers give new, correct products on every replay."""
+    import random
+    rng = random.Random(77)
+    sizes = [(64, 64), (100, 3000), (5000, 5000), (65536, 70), (20000, 33000), (9, 9)]
+    items, bufs = [], []
+    for na, nb in sizes:
+        da, db, do = fm.DeviceBuffer(na), fm.DeviceBuffer(nb), fm.DeviceBuffer(na + nb)
+        bufs += [da, db, do]
+        items.append((da, na, db, nb, do))
+    try:
+        with fm.Batch(items) as batch, fm.Stream() as s:
+            for rep in range(3):
+                ops = []
+                for (da, na, db, nb, do) in items:
+                    a = orc.splitmix(rng.getrandbits(40), na)
+                    b = orc.splitmix(rng.getrandbits(40), nb)
+                    da.upload(a)
+                    db.upload(b)
+                    ops.append((a, b))
+                batch.run(s)
+                s.synchronize()
+                for (a, b), (_, na, _, nb, do) in zip(ops, items):
+                    assert np.array_equal(do.download(na + nb), orc.mul(a, b)), (rep, na, nb)
+                # eager single calls in between replays
+                a, b = ops[2]
+                assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
+    finally:
+        for d in bufs:
+            d.free()
