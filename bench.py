@@ -93,6 +93,137 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, budget_s: float = 12.0):
     return prod, times
 
 
+def log_uniform_lengths(rng: np.random.Generator, count: int, lo: int, hi: int) -> np.ndarray:
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(np.int64)
+
+
+def run_batch(args, fm) -> dict:
+    """4,096 products, operand lengths log-uniform in [64, 65,536] limbs drawn
+    independently, 25 % squarings (sampler seed 4001), seeded-uniform limbs:
+    one planned batch (graph replay) against 4,096 plain device calls."""
+    count = 4096
+    rng = np.random.default_rng(4001)
+    l1 = log_uniform_lengths(rng, count, 64, 65536)
+    l2 = log_uniform_lengths(rng, count, 64, 65536)
+    sq = rng.random(count) < 0.25
+    stream = fm.Stream()
+    items, bufs = [], []
+    seed = 40_000
+    for i in range(count):
+        da = fm.DeviceBuffer.from_host(splitmix64(seed, int(l1[i])), stream)
+        db = None if sq[i] else fm.DeviceBuffer.from_host(splitmix64(seed + 1, int(l2[i])), stream)
+        seed += 2
+        n_out = 2 * int(l1[i]) if sq[i] else int(l1[i] + l2[i])
+        do = fm.DeviceBuffer(n_out)
+        bufs += [da, do] + ([db] if db is not None else [])
+        items.append((da, int(l1[i]), db, 0 if sq[i] else int(l2[i]), do))
+    limbs = int(sum(2 * l1[i] if sq[i] else l1[i] + l2[i] for i in range(count)))
+
+    def plain():
+        for (da, na, db, nb, do) in items:
+            if db is None:
+                fm.sqr_device(da, na, do, stream)
+            else:
+                fm.mul_device(da, na, db, nb, do, stream)
+
+    for _ in range(args.warmup):
+        plain()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.steps // 4)
+    for _ in range(reps):
+        plain()
+    stream.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    ref_out = [do.download(na + (nb if db is not None else na)) for (da, na, db, nb, do) in items[:64]]
+
+    batch = fm.Batch(items)
+    for _ in range(args.warmup):
+        batch.run(stream)
+    stream.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        batch.run(stream)
+        stream.synchronize()
+        lat.append(time.perf_counter() - s0)
+    t_batch = (time.perf_counter() - t0) / args.steps
+    for (da, na, db, nb, do), want in zip(items[:64], ref_out):
+        if not np.array_equal(do.download(want.size), want):
+            raise AssertionError("batched product differs from the plain call")
+    batch.close()
+    for d in bufs:
+        d.free()
+    stream.close()
+    return {
+        "metric": "mixed-size products/s (4,096 products, 64..65,536 limbs, 25% squares)",
+        "value": round(count / t_batch, 1),
+        "unit": "products/s",
+        "batch_ms": round(t_batch * 1e3, 3),
+        "batch_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "plain_calls_ms": round(t_plain * 1e3, 3),
+        "plain_products_per_s": round(count / t_plain, 1),
+        "speedup_vs_plain": round(t_plain / t_batch, 2),
+        "output_limbs": limbs,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "dtype": "u64",
+        "data": "synthetic (splitmix64, lengths log-uniform, sampler seed 4001)",
+        "config": {"workload": "batch of 4096 mixed-size products, hipGraph replay"},
+    }
+
+
+def run_sizes(args, fm) -> dict:
+    """Device-resident time of each single-product configuration."""
+    cases = [
+        ("mul 65536x65536", 65536, 65536, 1001),
+        ("sqr 65536", 65536, 0, 1003),
+        ("mul 1048576x1048576", 1 << 20, 1 << 20, 2001),
+        ("mul 4194304x262144", 4194304, 262144, 2003),
+        ("sqr 1048576", 1 << 20, 0, 2005),
+        ("mul all-ones 1048576x1048576", 1 << 20, 1 << 20, None),
+        ("mul 16777216x16777216", 1 << 24, 1 << 24, 3001),
+        ("sqr 16777216", 1 << 24, 0, 3003),
+    ]
+    stream = fm.Stream()
+    res = {}
+    for name, n1, n2, seed in cases:
+        if seed is None:
+            a = np.full(n1, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+            b = np.full(n2, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+        else:
+            a = splitmix64(seed, n1)
+            b = splitmix64(seed + 1, n2) if n2 else None
+        da = fm.DeviceBuffer.from_host(a, stream)
+        db = fm.DeviceBuffer.from_host(b, stream) if b is not None else None
+        do = fm.DeviceBuffer(n1 + (n2 if n2 else n1))
+
+        def one():
+            if db is None:
+                fm.sqr_device(da, n1, do, stream)
+            else:
+                fm.mul_device(da, n1, db, n2, do, stream)
+
+        for _ in range(args.warmup):
+            one()
+        stream.synchronize()
+        reps = max(3, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            one()
+        stream.synchronize()
+        res[name] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+        for d in (da, db, do):
+            if d is not None:
+                d.free()
+    stream.close()
+    return {"metric": "µs per product, device-resident", "unit": "µs", "values": res,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "u64",
+            "data": "synthetic (splitmix64, seeds 1000*cfg+k)", "config": {"workload": "single products"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +232,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")
+    ap.add_argument("--workload", default="mul1m", choices=["mul1m", "batch", "sizes"],
+                    help="mul1m: the 2^20 x 2^20 headline (default); batch: 4,096 mixed products; "
+                         "sizes: every single-product configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus})",
@@ -112,6 +246,12 @@ def main() -> int:
     L = fm.lib()
     if L.fmh_init(0) != 0:
         raise RuntimeError(L.fmh_last_error().decode())
+    if args.workload == "batch":
+        print(json.dumps(run_batch(args, fm)))
+        return 0
+    if args.workload == "sizes":
+        print(json.dumps(run_sizes(args, fm)))
+        return 0
     a = splitmix64(SEED_A, N1)
     b = splitmix64(SEED_B, N2)
     log_n = (N1 + N2 - 2).bit_length()

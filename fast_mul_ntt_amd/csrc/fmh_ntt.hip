@@ -538,9 +538,11 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
         s_line[threadIdx.x] = li;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const RowLine& li = s_line[e >> logL];
-        const u64 g = li.g0 + (u64)(e & ((1 << logL) - 1));
+    // Lines at least one workgroup long are walked line by line (the line's
+    // description is then uniform); shorter lines element by element.
+    const int L = 1 << logL;
+    const bool by_line = L >= (int)blockDim.x;
+    auto load = [&](const RowLine& li, u64 g, int e) {
         u64 va = 0, vb = 0;
         if (li.active) {
             if (A.from_limbs) {
@@ -553,6 +555,17 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
         }
         tA[e] = va;
         if (two) tB[e] = vb;
+    };
+    if (by_line) {
+        for (int c = 0; c < C; c++) {
+            const RowLine li = s_line[c];
+            for (int j = threadIdx.x; j < L; j += blockDim.x) load(li, li.g0 + (u64)j, (c << logL) + j);
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const RowLine& li = s_line[e >> logL];
+            load(li, li.g0 + (u64)(e & (L - 1)), e);
+        }
     }
     __syncthreads();
     if (two)
@@ -573,9 +586,17 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     }
     __syncthreads();
     tile_dit<P, 1>(tA, logL, logC, 0, 0u, A.itw[P]);
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const RowLine& li = s_line[e >> logL];
-        if (li.active) li.ra[li.g0 + (e & ((1 << logL) - 1))] = tA[e];
+    if (by_line) {
+        for (int c = 0; c < C; c++) {
+            const RowLine li = s_line[c];
+            if (!li.active) continue;
+            for (int j = threadIdx.x; j < L; j += blockDim.x) li.ra[li.g0 + j] = tA[(c << logL) + j];
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const RowLine& li = s_line[e >> logL];
+            if (li.active) li.ra[li.g0 + (e & (L - 1))] = tA[e];
+        }
     }
 }
 

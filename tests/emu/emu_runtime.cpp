@@ -3,6 +3,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <barrier>
 #include <chrono>
 #include <functional>
@@ -25,9 +26,15 @@ static size_t g_dyn_bytes = 0;
 
 void* emu_dyn_lds() { return g_dyn; }
 
-void emu_syncthreads() { g_bar->arrive_and_wait(); }
+static std::atomic<bool> g_used_barrier{false};
+
+void emu_syncthreads() {
+    g_used_barrier = true;
+    g_bar->arrive_and_wait();
+}
 
 unsigned emu_shfl_up_u32(unsigned v, unsigned delta, int width) {
+    g_used_barrier = true;
     const unsigned tid = threadIdx.x;
     g_slots[tid] = v;
     g_bar->arrive_and_wait();
@@ -49,6 +56,8 @@ struct emu_event {
 };
 
 // A team of block.x host threads runs every workgroup of one launch in turn.
+// A kernel whose first workgroup never synchronised is barrier-free: its other
+// workgroups run their threads one after the other on the calling thread.
 static void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, size_t lds) {
     if (block.x == 0 || block.x > 1024 || block.y != 1 || block.z != 1) {
         fprintf(stderr, "emu: bad block shape %u %u %u\n", block.x, block.y, block.z);
@@ -57,27 +66,65 @@ static void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, s
     gridDim = grid;
     blockDim = block;
     const unsigned long long nblocks = (unsigned long long)grid.x * grid.y * grid.z;
+    auto block_of = [&](unsigned long long b) {
+        return dim3((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
+                    (unsigned)(b / ((unsigned long long)grid.x * grid.y)));
+    };
+    auto new_lds = [&] {
+        // fresh dynamic LDS of exactly the requested size, filled with a pattern
+        g_dyn_bytes = lds;
+        g_dyn = lds ? malloc(lds) : nullptr;
+        if (lds) memset(g_dyn, 0xA5, lds);
+    };
     std::barrier<> bar(block.x);
-    std::barrier<> gate(block.x + 1);  // block start / end, with the launching thread
     g_bar = &bar;
+    g_used_barrier = false;
+    // first workgroup with real threads
+    new_lds();
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < block.x; t++)
+            th.emplace_back([&, t] {
+                threadIdx = dim3(t, 0, 0);
+                blockIdx = block_of(0);
+                body();
+            });
+        for (auto& h : th) h.join();
+    }
+    free(g_dyn);
+    g_dyn = nullptr;
+    if (nblocks == 1) return;
+    if (!g_used_barrier) {
+        const dim3 saved_t = threadIdx, saved_b = blockIdx;
+        for (unsigned long long b = 1; b < nblocks; b++) {
+            new_lds();
+            blockIdx = block_of(b);
+            for (unsigned t = 0; t < block.x; t++) {
+                threadIdx = dim3(t, 0, 0);
+                body();
+            }
+            free(g_dyn);
+            g_dyn = nullptr;
+        }
+        threadIdx = saved_t;
+        blockIdx = saved_b;
+        return;
+    }
+    std::barrier<> gate(block.x + 1);  // block start / end, with the launching thread
     std::vector<std::thread> team;
     team.reserve(block.x);
     for (unsigned t = 0; t < block.x; t++)
         team.emplace_back([&, t] {
             threadIdx = dim3(t, 0, 0);
-            for (unsigned long long b = 0; b < nblocks; b++) {
+            for (unsigned long long b = 1; b < nblocks; b++) {
                 gate.arrive_and_wait();  // block b starts
-                blockIdx = dim3((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
-                                (unsigned)(b / ((unsigned long long)grid.x * grid.y)));
+                blockIdx = block_of(b);
                 body();
                 gate.arrive_and_wait();  // block b done
             }
         });
-    for (unsigned long long b = 0; b < nblocks; b++) {
-        // fresh dynamic LDS of exactly the requested size, filled with a pattern
-        g_dyn_bytes = lds;
-        g_dyn = lds ? malloc(lds) : nullptr;
-        if (lds) memset(g_dyn, 0xA5, lds);
+    for (unsigned long long b = 1; b < nblocks; b++) {
+        new_lds();
         gate.arrive_and_wait();
         gate.arrive_and_wait();
         free(g_dyn);

@@ -200,3 +200,31 @@ def test_batch_graph_replays(fm, orc):
     finally:
         for d in bufs:
             d.free()
+
+
+def _primes61(count, seed):
+    import random
+    rng = random.Random(seed)
+    out = []
+    while len(out) < count:
+        q = rng.getrandbits(61) | (1 << 60) | 1
+        if all(pow(w, q - 1, q) == 1 for w in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)):
+            out.append(q)
+    return out
+
+
+@pytest.mark.parametrize("n1,n2", [(4194304, 262144), (1 << 24, 1 << 24), (1 << 24, 0)])
+def test_largest_products_mod_q(fm, orc, n1, n2):
+    """Full-size products (NTT 2^23 and 2^25; n2 == 0 is a square): residues mod
+    random 61-bit primes and the lowest limb must match the operands'."""
+    a = orc.splitmix(7 + n1, n1)
+    if n2:
+        b = orc.splitmix(9 + n2, n2)
+        got = fm.abs_mul64(a, b)
+    else:
+        b = a
+        got = fm.abs_sqr64(a)
+    assert got.size == n1 + (n2 or n1)
+    assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFFFFFFFFFF
+    for q in _primes61(3, n1 + n2):
+        assert orc.mod_q(got, q) == orc.mulmod_q(orc.mod_q(a, q), orc.mod_q(b, q), q), q
