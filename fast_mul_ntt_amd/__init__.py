@@ -100,6 +100,7 @@ def lib() -> ctypes.CDLL:
     L.fmh_batch_destroy.argtypes = [_vp]
     L.fmh_last_error.restype = ctypes.c_char_p
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
+    L.fmh_timing_only.argtypes = [ctypes.c_char_p]
     L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
     L.fmh_timing_collect.restype = _u64
     _LIB = L
@@ -291,7 +292,9 @@ class Batch:
         self.close()
 
 
-def timing_enable(on: bool) -> None:
+def timing_enable(on: bool, only: str | None = None) -> None:
+    """Per-kernel HIP-event timing; `only` restricts it to one kernel name."""
+    lib().fmh_timing_only(only.encode() if only else None)
     lib().fmh_timing_enable(1 if on else 0)
 
 

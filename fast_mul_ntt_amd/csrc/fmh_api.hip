@@ -416,6 +416,11 @@ int fmh_timing_enable(int on) {
     return 0;
 }
 
+int fmh_timing_only(const char* kernel) {
+    timing_only(kernel);
+    return 0;
+}
+
 u64 fmh_timing_collect(char* buf, u64 buflen) {
     const std::string s = timing_collect();
     if (buf && buflen) {

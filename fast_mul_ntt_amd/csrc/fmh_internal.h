@@ -134,7 +134,9 @@ std::string timing_collect();
 void timing_enable(bool on);
 bool timing_enabled();
 void timing_record(const char* name, hipEvent_t start, hipEvent_t stop);
-hipError_t timing_begin(hipStream_t s, hipEvent_t* start);
+// Restricts timing to kernels of one name (NULL or "" = all).
+void timing_only(const char* name);
+hipError_t timing_begin(const char* name, hipStream_t s, hipEvent_t* start);
 hipError_t timing_end(const char* name, hipStream_t s, hipEvent_t start);
 
 }  // namespace fmh

@@ -161,6 +161,7 @@ static int ceil_log2(u64 x) {
 
 static std::mutex g_tmu;
 static bool g_timing = false;
+static std::string g_timing_only;  // empty: time every kernel
 struct TimedLaunch {
     std::string name;
     hipEvent_t a, b;
@@ -175,9 +176,16 @@ bool timing_enabled() {
     std::lock_guard<std::mutex> lk(g_tmu);
     return g_timing;
 }
-hipError_t timing_begin(hipStream_t s, hipEvent_t* start) {
+void timing_only(const char* name) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing_only = name ? name : "";
+}
+hipError_t timing_begin(const char* name, hipStream_t s, hipEvent_t* start) {
     *start = nullptr;
-    if (!timing_enabled()) return hipSuccess;
+    {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        if (!g_timing || (!g_timing_only.empty() && g_timing_only != name)) return hipSuccess;
+    }
     hipError_t e = hipEventCreate(start);
     if (e != hipSuccess) return e;
     return hipEventRecord(*start, s);
@@ -229,7 +237,7 @@ std::string timing_collect() {
 #define FMH_LAUNCH(name, kern, grid, block, lds, stream, ...)                    \
     do {                                                                         \
         hipEvent_t ev_;                                                          \
-        hipError_t e_ = timing_begin(stream, &ev_);                              \
+        hipError_t e_ = timing_begin(name, stream, &ev_);                        \
         if (e_ != hipSuccess) return e_;                                         \
         hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);         \
         e_ = hipGetLastError();                                                  \

@@ -106,6 +106,9 @@ const char* fmh_last_error(void);
  * for the launches recorded since the last collect and returns the number of
  * bytes needed (including the terminating NUL). */
 int fmh_timing_enable(int on);
+/* Time only the kernels called `kernel` (NULL: all), so that the events add
+ * no gaps between the other kernels of a timed region. */
+int fmh_timing_only(const char* kernel);
 u64 fmh_timing_collect(char* buf, u64 buflen);
 
 #ifdef __cplusplus
