@@ -192,6 +192,132 @@ def run_batch(args, fm) -> dict:
     }
 
 
+def serve_requests(count: int = 20000, seed: int = 5001):
+    """Request stream: lengths log-uniform on [256, 4,194,304] limbs drawn
+    independently per operand, 20 % squarings, operand slices at seeded
+    offsets of one resident random limb array."""
+    rng = np.random.default_rng(seed)
+    l1 = log_uniform_lengths(rng, count, 256, 4194304)
+    l2 = log_uniform_lengths(rng, count, 256, 4194304)
+    sq = rng.random(count) < 0.20
+    src_len = 2 * 4194304
+    o1 = (rng.random(count) * (src_len - l1 + 1)).astype(np.int64)
+    o2 = (rng.random(count) * (src_len - l2 + 1)).astype(np.int64)
+    gaps = rng.exponential(1.0, size=count)  # unit-rate Poisson gaps, scaled by 1/lambda
+    return l1, l2, sq, o1, o2, gaps, src_len
+
+
+def run_serve(args, fm) -> dict:
+    """20,000 multiply/square requests through the request loop (64 GiB pool,
+    continuous batching), against plain per-request device calls."""
+    count = args.requests
+    l1, l2, sq, o1, o2, gaps, src_len = serve_requests(count)
+    stream = fm.Stream()
+    src = fm.DeviceBuffer.from_host(splitmix64(5001, src_len), stream)
+    ops = []
+    for i in range(count):
+        a = src.view(int(o1[i]), int(l1[i]))
+        b = None if sq[i] else src.view(int(o2[i]), int(l2[i]))
+        ops.append((a, b))
+    out_max = int(max(2 * l1[i] if sq[i] else l1[i] + l2[i] for i in range(count)))
+    out = fm.DeviceBuffer(out_max)
+
+    def plain(i):
+        a, b = ops[i]
+        if b is None:
+            fm.sqr_device(a, a.n, out, stream)
+        else:
+            fm.mul_device(a, a.n, b, b.n, out, stream)
+
+    for i in range(min(8, count)):
+        plain(i)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for i in range(count):
+        plain(i)
+    stream.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain_rate = count / t_plain
+
+    srv = fm.Server(pool_bytes=args.pool_gib << 30, streams=4)
+    check_every = max(1, count // 40)
+    checked = {}
+
+    def drive(rate: float | None):
+        """Submit every request (at Poisson arrival times if rate is given),
+        releasing outputs as they complete; returns submission lateness (s)."""
+        srv.reset_stats()
+        late = []
+        ids = {}
+        t_start = time.perf_counter()
+        due = t_start + (np.cumsum(gaps) / rate if rate else np.zeros(count))
+        i = 0
+        outstanding = 0
+        while i < count or outstanding:
+            now = time.perf_counter()
+            while i < count and due[i] <= now:
+                a, b = ops[i]
+                rid = srv.submit(a, a.n, b, 0 if b is None else b.n)
+                late.append(time.perf_counter() - due[i])
+                ids[rid] = i
+                i += 1
+                outstanding += 1
+            for rid in srv.poll():
+                k = ids.pop(rid)
+                if rate and k % check_every == 0:
+                    checked[k] = srv.result(rid)
+                srv.release(rid)
+                outstanding -= 1
+            if i < count and due[i] > time.perf_counter() + 2e-4:
+                time.sleep(1e-4)
+        return late
+
+    drive(None)  # closed loop: capacity
+    cap = srv.stats()
+    rate = args.load * cap["products_per_s"]
+    late = drive(rate)
+    st = srv.stats()
+    srv.close()
+    # served outputs equal the products computed alone
+    for k, got in checked.items():
+        plain(k)
+        stream.synchronize()
+        if not np.array_equal(out.download(got.size), got):
+            raise AssertionError(f"served request {k} differs from the plain call")
+    out.free()
+    src.free()
+    stream.close()
+    limbs = int(sum(2 * l1[i] if sq[i] else l1[i] + l2[i] for i in range(count)))
+    return {
+        "metric": "served products/s (Poisson arrivals) + p99 latency",
+        "value": round(st["products_per_s"], 1),
+        "unit": "products/s",
+        "p50_ms": round(st["p50_us"] / 1e3, 3),
+        "p99_ms": round(st["p99_us"] / 1e3, 3),
+        "max_ms": round(st["max_us"] / 1e3, 3),
+        "offered_rate": round(rate, 1),
+        "load": args.load,
+        "capacity_products_per_s": round(cap["products_per_s"], 1),
+        "capacity_p99_ms": round(cap["p99_us"] / 1e3, 3),
+        "plain_products_per_s": round(plain_rate, 1),
+        "speedup_vs_plain": round(st["products_per_s"] / plain_rate, 2),
+        "capacity_vs_plain": round(cap["products_per_s"] / plain_rate, 2),
+        "mean_group": round(st["mean_group"], 2),
+        "groups": st["groups"],
+        "pool_gib": args.pool_gib,
+        "pool_peak_gib": round(st["pool_peak_bytes"] / 2**30, 3),
+        "submit_lateness_p99_ms": round(float(np.percentile(late, 99)) * 1e3, 3),
+        "checked_vs_plain": len(checked),
+        "requests": count,
+        "output_limbs": limbs,
+        "higher_is_better": True,
+        "dtype": "u64",
+        "data": "synthetic (splitmix64 seed 5001 source array, lengths/offsets/arrivals seed 5001)",
+        "config": {"workload": f"serve {count} requests, 256..4194304 limbs, 20% squares, "
+                               f"Poisson arrivals at {args.load:.0%} of capacity"},
+    }
+
+
 def run_sizes(args, fm) -> dict:
     """Device-resident time of each single-product configuration."""
     cases = [
@@ -249,9 +375,13 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")
-    ap.add_argument("--workload", default="mul1m", choices=["mul1m", "batch", "sizes"],
+    ap.add_argument("--workload", default="mul1m", choices=["mul1m", "batch", "sizes", "serve"],
                     help="mul1m: the 2^20 x 2^20 headline (default); batch: 4,096 mixed products; "
-                         "sizes: every single-product configuration")
+                         "sizes: every single-product configuration; serve: 20,000 requests "
+                         "through the request loop")
+    ap.add_argument("--requests", type=int, default=20000, help="serve: number of requests")
+    ap.add_argument("--load", type=float, default=0.7, help="serve: offered rate / capacity")
+    ap.add_argument("--pool-gib", type=int, default=64, help="serve: device pool (GiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus})",
@@ -268,6 +398,9 @@ def main() -> int:
         return 0
     if args.workload == "sizes":
         print(json.dumps(run_sizes(args, fm)))
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args, fm)))
         return 0
     a = splitmix64(SEED_A, N1)
     b = splitmix64(SEED_B, N2)

@@ -28,6 +28,7 @@ __all__ = [
     "abs_mul64",
     "abs_sqr64",
     "DeviceBuffer",
+    "DeviceView",
     "Stream",
     "mul_device",
     "sqr_device",
@@ -36,6 +37,10 @@ __all__ = [
     "FmhProduct",
     "mul_batch_device",
     "Batch",
+    "Spectrum",
+    "ServerConfig",
+    "ServerInfo",
+    "Server",
     "timing_enable",
     "timing_collect",
     "FmhError",
@@ -59,6 +64,26 @@ class FmhProduct(ctypes.Structure):
     """struct fmh_product of include/fmh.h (device pointers)."""
 
     _fields_ = [("in1", _vp), ("len1", _u64), ("in2", _vp), ("len2", _u64), ("out", _vp)]
+
+
+class ServerConfig(ctypes.Structure):
+    """fmh_server_config of include/fmh.h (zeros pick the defaults)."""
+
+    _fields_ = [("pool_bytes", _u64), ("streams", ctypes.c_uint), ("max_group", ctypes.c_uint),
+                ("max_group_bytes", _u64)]
+
+
+class ServerInfo(ctypes.Structure):
+    """fmh_server_info of include/fmh.h."""
+
+    _fields_ = [("submitted", _u64), ("completed", _u64), ("failed", _u64), ("groups", _u64),
+                ("mean_group", ctypes.c_double), ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+                ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double),
+                ("products_per_s", ctypes.c_double), ("span_s", ctypes.c_double),
+                ("pool_bytes", _u64), ("pool_peak_bytes", _u64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def declared_symbols(header: str = HEADER_PATH) -> list[str]:
@@ -98,6 +123,26 @@ def lib() -> ctypes.CDLL:
     L.fmh_batch_create.argtypes = [ctypes.POINTER(FmhProduct), _u64, ctypes.POINTER(_vp)]
     L.fmh_batch_run.argtypes = [_vp, _vp]
     L.fmh_batch_destroy.argtypes = [_vp]
+    L.fmh_spectrum_create.argtypes = [_vp, _u64, _u64, _vp, ctypes.POINTER(_vp)]
+    L.fmh_spectrum_ntt_len.argtypes = [_vp]
+    L.fmh_spectrum_ntt_len.restype = _u64
+    L.fmh_mul_spectrum_device.argtypes = [_vp, _vp, _u64, _vp, _vp]
+    L.fmh_sqr_spectrum_device.argtypes = [_vp, _vp, _vp]
+    L.fmh_spectrum_destroy.argtypes = [_vp]
+    L.fmh_spectrum_describe.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_u64),
+                                        ctypes.POINTER(ctypes.c_int)]
+    L.fmh_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+    L.fmh_server_submit.argtypes = [_vp, _vp, _u64, _vp, _u64, ctypes.POINTER(_u64)]
+    L.fmh_server_submit_spectrum.argtypes = [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+    L.fmh_server_poll.argtypes = [_vp, ctypes.POINTER(_u64), _u64, ctypes.POINTER(_u64)]
+    L.fmh_server_wait.argtypes = [_vp, _u64]
+    L.fmh_server_result.argtypes = [_vp, _u64, ctypes.POINTER(_vp), ctypes.POINTER(_u64)]
+    L.fmh_server_release.argtypes = [_vp, _u64]
+    L.fmh_server_drain.argtypes = [_vp]
+    L.fmh_server_stats.argtypes = [_vp, ctypes.POINTER(ServerInfo)]
+    L.fmh_server_reset_stats.argtypes = [_vp]
+    L.fmh_server_destroy.argtypes = [_vp]
+    L.fmh_server_last_error.restype = ctypes.c_char_p
     L.fmh_last_error.restype = ctypes.c_char_p
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
     L.fmh_timing_only.argtypes = [ctypes.c_char_p]
@@ -202,6 +247,12 @@ class DeviceBuffer:
                                         stream.handle if stream else None), "fmh_memcpy_d2h")
         return out
 
+    def view(self, offset: int, n: int) -> "DeviceView":
+        """Non-owning view of limbs [offset, offset + n)."""
+        if offset < 0 or n < 0 or offset + n > self.n:
+            raise ValueError("view outside the buffer")
+        return DeviceView(self.ptr + 8 * offset, n)
+
     def free(self) -> None:
         if self.ptr is not None:
             _check(lib().fmh_dev_free(self.ptr), "fmh_dev_free")
@@ -212,6 +263,21 @@ class DeviceBuffer:
 
     def __exit__(self, *exc):
         self.free()
+
+
+class DeviceView:
+    """Limbs inside a DeviceBuffer (pointer and length; owns nothing)."""
+
+    def __init__(self, ptr: int, n: int):
+        self.ptr = ptr
+        self.n = int(n)
+
+    def download(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=np.uint64)
+        if n:
+            _check(lib().fmh_memcpy_d2h(out.ctypes.data, self.ptr, n * 8, None), "fmh_memcpy_d2h")
+        return out
 
 
 def mul_device(a: DeviceBuffer, na: int, b: DeviceBuffer, nb: int, out: DeviceBuffer,
@@ -283,6 +349,134 @@ class Batch:
     def close(self) -> None:
         if self.handle is not None:
             _check(lib().fmh_batch_destroy(self.handle), "fmh_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Spectrum:
+    """Resident forward transform of one operand (src/fast_mul.c:135-212
+    conv_single_i semantics): products with it transform only the other
+    operand, squares of it transform nothing."""
+
+    def __init__(self, a: DeviceBuffer, na: int, ntt_len: int = 0, stream: Stream | None = None):
+        if a.n < na:
+            raise ValueError("device buffer too small")
+        h = _vp()
+        _check(lib().fmh_spectrum_create(a.ptr, na, ntt_len, stream.handle if stream else None,
+                                         ctypes.byref(h)), "fmh_spectrum_create")
+        self.handle = h.value
+        self.len = int(na)
+        self.ntt_len = int(lib().fmh_spectrum_ntt_len(self.handle))
+
+    def mul(self, b: DeviceBuffer, nb: int, out: DeviceBuffer, stream: Stream | None = None) -> None:
+        """Enqueue out = (operand of the spectrum) * b[:nb]."""
+        if b.n < nb or out.n < self.len + nb:
+            raise ValueError("device buffers too small")
+        _check(lib().fmh_mul_spectrum_device(self.handle, b.ptr, nb, out.ptr,
+                                             stream.handle if stream else None), "fmh_mul_spectrum_device")
+
+    def sqr(self, out: DeviceBuffer, stream: Stream | None = None) -> None:
+        """Enqueue out = (operand of the spectrum) ** 2."""
+        if out.n < 2 * self.len:
+            raise ValueError("device buffer too small")
+        _check(lib().fmh_sqr_spectrum_device(self.handle, out.ptr, stream.handle if stream else None),
+               "fmh_sqr_spectrum_device")
+
+    def close(self) -> None:
+        if self.handle is not None:
+            _check(lib().fmh_spectrum_destroy(self.handle), "fmh_spectrum_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _check_srv(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().fmh_server_last_error()
+        raise FmhError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Server:
+    """Request loop with continuous batching over a device pool (fmh_server_*)."""
+
+    def __init__(self, pool_bytes: int = 0, streams: int = 0, max_group: int = 0, max_group_bytes: int = 0):
+        cfg = ServerConfig(pool_bytes, streams, max_group, max_group_bytes)
+        h = _vp()
+        _check_srv(lib().fmh_server_create(ctypes.byref(cfg), ctypes.byref(h)), "fmh_server_create")
+        self.handle = h.value
+        self._ids = (_u64 * 4096)()
+
+    def submit(self, a: DeviceBuffer, na: int, b: DeviceBuffer | None = None, nb: int = 0) -> int:
+        """Request a[:na] * b[:nb] (b None: the square); returns its id."""
+        if a.n < na or (b is not None and b.n < nb):
+            raise ValueError("device buffers too small")
+        rid = _u64()
+        _check_srv(lib().fmh_server_submit(self.handle, a.ptr, na, None if b is None else b.ptr,
+                                           nb if b is not None else 0, ctypes.byref(rid)), "fmh_server_submit")
+        return rid.value
+
+    def submit_spectrum(self, spec: Spectrum, b: DeviceBuffer | None = None, nb: int = 0) -> int:
+        """Request (operand of spec) * b[:nb] (b None: its square); returns its id."""
+        rid = _u64()
+        _check_srv(lib().fmh_server_submit_spectrum(self.handle, spec.handle, None if b is None else b.ptr,
+                                                    nb if b is not None else 0, ctypes.byref(rid)),
+                   "fmh_server_submit_spectrum")
+        return rid.value
+
+    def poll(self) -> list[int]:
+        """Ids completed (or failed) since the last poll."""
+        out = []
+        while True:
+            n = _u64()
+            _check_srv(lib().fmh_server_poll(self.handle, self._ids, len(self._ids), ctypes.byref(n)),
+                       "fmh_server_poll")
+            out.extend(self._ids[i] for i in range(n.value))
+            if n.value < len(self._ids):
+                return out
+
+    def wait(self, rid: int) -> None:
+        _check_srv(lib().fmh_server_wait(self.handle, rid), "fmh_server_wait")
+
+    def result_ptr(self, rid: int) -> tuple[int, int]:
+        """(device pointer, limbs) of a completed request's output."""
+        p, n = _vp(), _u64()
+        _check_srv(lib().fmh_server_result(self.handle, rid, ctypes.byref(p), ctypes.byref(n)),
+                   "fmh_server_result")
+        return p.value, n.value
+
+    def result(self, rid: int) -> np.ndarray:
+        """Copy of a completed request's output (host array)."""
+        p, n = self.result_ptr(rid)
+        out = np.empty(n, dtype=np.uint64)
+        _check(lib().fmh_memcpy_d2h(out.ctypes.data, p, n * 8, None), "fmh_memcpy_d2h")
+        return out
+
+    def release(self, rid: int) -> None:
+        _check_srv(lib().fmh_server_release(self.handle, rid), "fmh_server_release")
+
+    def drain(self) -> None:
+        _check_srv(lib().fmh_server_drain(self.handle), "fmh_server_drain")
+
+    def stats(self) -> dict:
+        st = ServerInfo()
+        _check_srv(lib().fmh_server_stats(self.handle, ctypes.byref(st)), "fmh_server_stats")
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        _check_srv(lib().fmh_server_reset_stats(self.handle), "fmh_server_reset_stats")
+
+    def close(self) -> None:
+        if self.handle is not None:
+            _check_srv(lib().fmh_server_destroy(self.handle), "fmh_server_destroy")
             self.handle = None
 
     def __enter__(self):

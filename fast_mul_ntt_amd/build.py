@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fmh_ntt.hip", "fmh_api.hip"]
+SOURCES = ["fmh_ntt.hip", "fmh_api.hip", "fmh_serve.hip"]
 OUTPUT = os.path.join(HERE, "libfmh.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

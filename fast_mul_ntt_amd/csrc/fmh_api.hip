@@ -117,7 +117,9 @@ std::map<hipStream_t, Staging> g_stage;
 // Enqueue a set of products on stream s; callers hold g_mu.
 int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
     Job job;
-    if (job_build(job, in, count)) return set_err("product longer than fmh_max_ntt_len()");
+    const int rc = job_build(job, in, count);
+    if (rc == -2) return set_err("product longer than the spectrum's transform length");
+    if (rc) return set_err("product longer than fmh_max_ntt_len()");
     if (ensure_twiddles(c, job.max_log_n)) return -1;
     Buffer& w = c.ws[s];
     if (ensure_buffer(w, job_ws_bytes(job), s)) return -1;
@@ -210,6 +212,21 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
 
 }  // namespace
 
+namespace fmh {
+
+int device_twiddles(int log_n, Twiddles* out, std::string* err) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c) || ensure_twiddles(*c, log_n)) {
+        if (err) *err = t_err;
+        return -1;
+    }
+    *out = c->tw;
+    return 0;
+}
+
+}  // namespace fmh
+
 extern "C" {
 
 void abs_mul64(const u64* in1, u64 len1, const u64* in2, u64 len2, u64* out) {
@@ -265,6 +282,93 @@ int fmh_mul_batch_device(const fmh_product* items, u64 count, fmh_stream_t strea
     DevCtx* c = nullptr;
     if (ctx_get(&c)) return -1;
     return run_job(*c, in.data(), in.size(), (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+struct fmh_spectrum {
+    u64* words = nullptr;  // 3 x 2^log_n
+    u64 len = 0;           // limbs of the transformed operand
+    int log_n = 0;
+    int device = 0;
+};
+
+extern "C" {
+
+int fmh_spectrum_create(const u64* d_in, u64 len, u64 ntt_len, fmh_stream_t stream, fmh_spectrum** out) {
+    if (!out) return set_err("NULL argument");
+    *out = nullptr;
+    if (len == 0) return set_err("an empty operand has no spectrum");
+    if (ntt_len == 0) ntt_len = 1ull << ceil_log2(2 * len - 1);
+    if ((ntt_len & (ntt_len - 1)) || ntt_len < len) return set_err("ntt_len must be a power of two >= len");
+    const int log_n = ceil_log2(ntt_len);
+    if (log_n > kMaxLogN) return set_err("ntt_len above fmh_max_ntt_len()");
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    if (ensure_twiddles(*c, log_n)) return -1;
+    fmh_spectrum* sp = new fmh_spectrum();
+    sp->len = len;
+    sp->log_n = log_n;
+    (void)hipGetDevice(&sp->device);
+    hipError_t e = hipMalloc(&sp->words, (size_t)3 * sizeof(u64) << log_n);
+    if (e != hipSuccess) {
+        delete sp;
+        return set_err("hipMalloc", e);
+    }
+    e = enqueue_resident_spectrum(d_in, len, log_n, sp->words, c->tw, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        (void)hipFree(sp->words);
+        delete sp;
+        return set_err("kernel launch", e);
+    }
+    *out = sp;
+    return 0;
+}
+
+u64 fmh_spectrum_ntt_len(const fmh_spectrum* sp) { return sp ? 1ull << sp->log_n : 0; }
+
+int fmh_spectrum_describe(const fmh_spectrum* sp, const u64** words, u64* len, int* log_n) {
+    if (!sp) return set_err("NULL spectrum");
+    *words = sp->words;
+    *len = sp->len;
+    *log_n = sp->log_n;
+    return 0;
+}
+
+int fmh_mul_spectrum_device(const fmh_spectrum* sp, const u64* d_in2, u64 len2, u64* d_out,
+                            fmh_stream_t stream) {
+    if (!sp) return set_err("NULL spectrum");
+    if (len2 && !d_in2) return set_err("NULL operand");
+    if (len2 && sp->len + len2 - 1 > (1ull << sp->log_n))
+        return set_err("product longer than the spectrum's transform length");
+    ProductIn in{d_in2, len2, nullptr, sp->len, d_out, sp->words, sp->log_n};
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    if (len2 == 0) {
+        hipError_t e = enqueue_zero(d_out, sp->len, (hipStream_t)stream);
+        return e == hipSuccess ? 0 : set_err("kernel launch", e);
+    }
+    return run_job(*c, &in, 1, (hipStream_t)stream);
+}
+
+int fmh_sqr_spectrum_device(const fmh_spectrum* sp, u64* d_out, fmh_stream_t stream) {
+    if (!sp) return set_err("NULL spectrum");
+    if (2 * sp->len - 1 > (1ull << sp->log_n)) return set_err("square longer than the spectrum's transform length");
+    ProductIn in{nullptr, 0, nullptr, sp->len, d_out, sp->words, sp->log_n};
+    std::lock_guard<std::mutex> lk(g_mu);
+    DevCtx* c = nullptr;
+    if (ctx_get(&c)) return -1;
+    return run_job(*c, &in, 1, (hipStream_t)stream);
+}
+
+int fmh_spectrum_destroy(fmh_spectrum* sp) {
+    if (!sp) return 0;
+    hipError_t e = hipDeviceSynchronize();
+    (void)hipFree(sp->words);
+    delete sp;
+    return e == hipSuccess ? 0 : set_err("hipDeviceSynchronize", e);
 }
 
 }  // extern "C"

@@ -63,35 +63,51 @@ struct Twiddles {
 
 hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s);
 
+// What a product transforms.  A resident spectrum is the complete forward
+// transform of an operand (3 arrays of 2^log_n words, values in [0, p), the
+// layout the row pass produces), kept in device memory for reuse.
+enum ProductKind : u32 {
+    kKindMul = 0,     // a * b: both operands transformed
+    kKindSqr = 1,     // a * a: one forward transform
+    kKindMulSpec = 2, // a * (operand of spec): a transformed, spec read
+    kKindSqrSpec = 3, // (operand of spec)^2: no forward transform
+};
+
 // One product as launched: device pointers and its share of the workspace.
 struct ProductDesc {
     const u64* a;
-    const u64* b;  // equals a for a square
-    u64 na, nb;
-    u64* out;      // na + nb limbs
-    u64* res;      // residues: 3 (square) or 6 arrays of 2^log_n words
+    const u64* b;     // equals a for a square
+    const u64* spec;  // resident spectrum (kinds kKindMulSpec, kKindSqrSpec)
+    u64 na, nb;       // nb: length of the spectrum's operand for the spec kinds
+    u64* out;         // na + nb limbs
+    u64* res;         // residues: 3 arrays of 2^log_n words (6 for kKindMul)
     u32 log_n;
-    u32 square;
-    u32 chunk0;    // first carry chunk of this product in the job's chunk arrays
+    u32 kind;
+    u32 chunk0;       // first carry chunk of this product in the job's chunk arrays
     u32 nchunks;
     CrtConsts crt;
 };
 
-// A product as requested: b == nullptr (or b == a with nb == na) is a square.
+// A product as requested.  Without a spectrum: b == nullptr (or b == a with
+// nb == na) is a square.  With spec (a resident spectrum of 2^spec_log_n
+// points of an operand of nb limbs): a == nullptr is the square of that
+// operand, otherwise the product a * that operand.
 struct ProductIn {
     const u64* a;
     u64 na;
     const u64* b;
     u64 nb;
     u64* out;
+    const u64* spec = nullptr;
+    int spec_log_n = 0;
 };
 
 // A set of products launched together: one product, or a batch of products
-// of any sizes.  Products are grouped into size classes (square flag, log_n)
+// of any sizes.  Products are grouped into size classes (kind, log_n)
 // and each class runs as one launch per pass; the CRT and carry kernels cover
 // all products at once.
 struct Job {
-    std::vector<ProductDesc> items;  // sorted by (square, log_n)
+    std::vector<ProductDesc> items;  // sorted by (kind, log_n)
     std::vector<u64> res_off;        // residue offset of each item (words)
     std::vector<u32> chunk_item;     // product index of each carry chunk
     std::vector<std::pair<u64*, u64>> zeros;  // outputs of products with an empty operand
@@ -109,7 +125,8 @@ struct Job {
 // One-time kernel attributes and constant tables (call before graph capture).
 void prepare_kernels();
 
-// Fills the job; returns 0, or -1 if a product is longer than 2^kMaxLogN.
+// Fills the job; returns 0, -1 if a product is longer than 2^kMaxLogN, or -2
+// if a product does not fit the transform length of its resident spectrum.
 int job_build(Job& j, const ProductIn* in, size_t count);
 size_t job_ws_bytes(const Job& j);
 void job_bind(Job& j, void* ws);
@@ -126,6 +143,15 @@ hipError_t enqueue_copy(u64* dst, const u64* src, u64 n, hipStream_t s);
 size_t spectrum_ws_bytes(int log_n);
 hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
                             const Twiddles& tw, hipStream_t s);
+// Resident spectrum of an operand: the 3 x 2^log_n words at spec (no other
+// workspace).
+hipError_t enqueue_resident_spectrum(const u64* a, u64 na, int log_n, u64* spec, const Twiddles& tw,
+                                     hipStream_t s);
+
+// Twiddle tables of the current device covering transforms of 2^log_n points
+// (built on first use; tables are never freed, so a copy stays valid).
+// Returns 0, or -1 with a message in *err.
+int device_twiddles(int log_n, Twiddles* out, std::string* err);
 
 // Optional per-kernel timing with HIP events on the launch stream.
 // timing_collect() waits for the recorded launches and returns one

@@ -492,7 +492,13 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
 // for small products (s0 = 0) every line is a whole product, so one
 // workgroup carries C small transforms at once.
 
-enum RowMode { kRowMul = 0, kRowSqr = 1, kRowFwdOnly = 2 };
+// Row modes: kRowMul, kRowSqr and kRowMulSpec / kRowSqrSpec (the product
+// kinds of the same names) finish the forward transform of the loaded
+// operand(s), form the pointwise product and run the first inverse stages;
+// kRowFwdOnly exports the spectrum in canonical Montgomery form (parity
+// checks); kRowFwdKeep stores it as the plain values [0, p) of a resident
+// spectrum.
+enum RowMode { kRowMul = 0, kRowSqr = 1, kRowMulSpec = 2, kRowSqrSpec = 3, kRowFwdOnly = 4, kRowFwdKeep = 5 };
 constexpr int kMaxRowLines = 64;
 
 struct RowArgs {
@@ -514,7 +520,8 @@ struct RowLine {
     u64 na, nb;
     u64* ra;
     const u64* rb;
-    u64 g0;  // offset of the line inside its product's arrays
+    const u64* rs;  // resident spectrum of this prime (spec modes)
+    u64 g0;         // offset of the line inside its product's arrays
     u32 active;
 };
 
@@ -526,6 +533,7 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     const int C = 1 << logC;
     const int n = C << logL;
     const bool two = (A.mode == kRowMul);
+    const bool spec_in = (A.mode == kRowSqrSpec);  // the tile starts as a spectrum
     u64* __restrict__ tA = lds;
     u64* __restrict__ tB = lds + n;
     if ((int)threadIdx.x < C) {
@@ -541,6 +549,7 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
             li.nb = D.nb;
             li.ra = D.res + ((u64)P << A.log_n);
             li.rb = D.res + ((u64)(3 + P) << A.log_n);
+            li.rs = D.spec ? D.spec + ((u64)P << A.log_n) : nullptr;
             li.g0 = (lambda & ((1ull << A.s0) - 1)) << logL;
         }
         s_line[threadIdx.x] = li;
@@ -553,7 +562,9 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     auto load = [&](const RowLine& li, u64 g, int e) {
         u64 va = 0, vb = 0;
         if (li.active) {
-            if (A.from_limbs) {
+            if (spec_in) {
+                va = li.rs[g];
+            } else if (A.from_limbs) {
                 va = g < li.na ? reduce_word<P>(li.a[g]) : 0ull;
                 if (two) vb = g < li.nb ? reduce_word<P>(li.b[g]) : 0ull;
             } else {
@@ -578,19 +589,29 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     __syncthreads();
     if (two)
         tile_dif<P, 1, 2>(tA, tB, logL, logC, 0, 0u, A.tw[P]);
-    else
+    else if (!spec_in)
         tile_dif<P, 1, 1>(tA, nullptr, logL, logC, 0, 0u, A.tw[P]);
-    if (A.mode == kRowFwdOnly) {
+    if (A.mode == kRowFwdOnly || A.mode == kRowFwdKeep) {
         const u64 r2 = A.r2[P];
+        const bool mont = A.mode == kRowFwdOnly;
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
             const RowLine& li = s_line[e >> logL];
-            if (li.active) li.ra[li.g0 + (e & ((1 << logL) - 1))] = canon<P>(redc<P>(tA[e], r2));
+            if (li.active)
+                li.ra[li.g0 + (e & ((1 << logL) - 1))] = canon<P>(mont ? redc<P>(tA[e], r2) : norm2p<P>(tA[e]));
         }
         return;
     }
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const u64 x = tA[e];
-        tA[e] = redc<P>(x, two ? tB[e] : x);
+    if (A.mode == kRowMulSpec) {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const RowLine& li = s_line[e >> logL];
+            const u64 y = li.active ? li.rs[li.g0 + (e & ((1 << logL) - 1))] : 0ull;
+            tA[e] = redc<P>(tA[e], y);
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const u64 x = tA[e];
+            tA[e] = redc<P>(x, two ? tB[e] : x);
+        }
     }
     __syncthreads();
     tile_dit<P, 1>(tA, logL, logC, 0, 0u, A.itw[P]);
@@ -994,35 +1015,49 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
     j = Job();
     for (size_t i = 0; i < count; i++) {
         const ProductIn& p = in[i];
-        const bool square = p.b == nullptr || (p.b == p.a && p.nb == p.na);
-        const u64 nb = square ? p.na : p.nb;
-        if (p.na == 0 || nb == 0) {
-            if (p.na + nb) j.zeros.push_back({p.out, p.na + nb});
+        ProductDesc d{};
+        if (p.spec) {
+            d.kind = p.a ? kKindMulSpec : kKindSqrSpec;
+            d.a = p.a ? p.a : nullptr;
+            d.b = nullptr;
+            d.spec = p.spec;
+            d.na = p.a ? p.na : p.nb;
+            d.nb = p.nb;
+        } else {
+            const bool square = p.b == nullptr || (p.b == p.a && p.nb == p.na);
+            d.kind = square ? kKindSqr : kKindMul;
+            d.a = p.a;
+            d.b = square ? p.a : p.b;
+            d.na = p.na;
+            d.nb = square ? p.na : p.nb;
+        }
+        d.out = p.out;
+        if (d.na == 0 || d.nb == 0) {
+            if (d.na + d.nb) j.zeros.push_back({p.out, d.na + d.nb});
             continue;
         }
-        const u64 out_len = p.na + nb;
-        const int log_n = ceil_log2(out_len - 1);
+        const u64 out_len = d.na + d.nb;
+        int log_n = ceil_log2(out_len - 1);
         if (log_n > kMaxLogN) return -1;
-        ProductDesc d{};
-        d.a = p.a;
-        d.b = square ? p.a : p.b;
-        d.na = p.na;
-        d.nb = nb;
-        d.out = p.out;
+        if (p.spec) {
+            // the product is taken at the spectrum's length (any cyclic
+            // length >= na + nb - 1 gives the same coefficients)
+            if (log_n > p.spec_log_n) return -2;
+            log_n = p.spec_log_n;
+        }
         d.log_n = (u32)log_n;
-        d.square = square ? 1u : 0u;
         d.nchunks = (u32)((out_len + kCarryChunk - 1) / kCarryChunk);
         d.crt = crt_table(log_n);
         j.items.push_back(d);
     }
     std::stable_sort(j.items.begin(), j.items.end(), [](const ProductDesc& x, const ProductDesc& y) {
-        return x.square != y.square ? x.square < y.square : x.log_n < y.log_n;
+        return x.kind != y.kind ? x.kind < y.kind : x.log_n < y.log_n;
     });
     j.res_off.resize(j.items.size());
     for (size_t i = 0; i < j.items.size(); i++) {
         ProductDesc& d = j.items[i];
         j.res_off[i] = j.res_words;
-        j.res_words += (u64)(d.square ? 3 : 6) << d.log_n;
+        j.res_words += (u64)(d.kind == kKindMul ? 6 : 3) << d.log_n;
         d.chunk0 = j.chunks;
         j.chunks += d.nchunks;
         for (u32 c = 0; c < d.nchunks; c++) j.chunk_item.push_back((u32)i);
@@ -1080,12 +1115,12 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     const Tuning& tu = tuning();
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
-    const bool square = d0.square != 0;
+    const u32 kind = d0.kind;
     const u64 N = 1ull << log_n;
     const Plan pl = make_plan(log_n);
     const ProductDesc* items = j.d_items ? j.d_items + first : nullptr;
-    const int nops = square ? 1 : 2;
-    for (int pi = 0; pi < pl.n_col; pi++) {
+    const int nops = kind == kKindMul ? 2 : kind == kKindSqrSpec ? 0 : 1;  // operands to transform
+    for (int pi = 0; nops > 0 && pi < pl.n_col; pi++) {
         ColArgs C{};
         C.items = items;
         C.one = d0;
@@ -1115,17 +1150,18 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         R.log_n = log_n;
         R.s0 = pl.row_s;
         R.from_limbs = (pl.n_col == 0);
-        R.mode = square ? kRowSqr : kRowMul;
+        static const char* const kRowName[4] = {"row_mul", "row_sqr", "row_mul_spec", "row_sqr_spec"};
+        static const int kRowModeOf[4] = {kRowMul, kRowSqr, kRowMulSpec, kRowSqrSpec};
+        R.mode = kRowModeOf[kind];
         const int logL = log_n - pl.row_s;
         const u64 lines = (u64)count << pl.row_s;
         int logC = std::max(0, std::min(tu.row_tile - logL, 6));  // kMaxRowLines = 64
         while (logC > 0 && (1ull << (logC - 1)) >= lines) logC--;
         R.logC = logC;
         const u64 tiles = (lines + (1ull << logC) - 1) >> logC;
-        const size_t lds = (size_t)(square ? 1 : 2) * ((size_t)1 << (logL + logC)) * sizeof(u64);
+        const size_t lds = (size_t)(kind == kKindMul ? 2 : 1) * ((size_t)1 << (logL + logC)) * sizeof(u64);
         if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
-        FMH_LAUNCH(square ? "row_sqr" : "row_mul", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds,
-                   s, R);
+        FMH_LAUNCH(kRowName[kind], k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
     }
     for (int pi = pl.n_col - 1; pi >= 0; pi--) {
         ColArgs C{};
@@ -1161,7 +1197,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     while (first < j.items.size()) {
         size_t last = first;
         while (last < j.items.size() && last - first < kMaxClass && j.items[last].log_n == j.items[first].log_n &&
-               j.items[last].square == j.items[first].square)
+               j.items[last].kind == j.items[first].kind)
             last++;
         hipError_t e = launch_class(j, first, (u32)(last - first), tw, s);
         if (e != hipSuccess) return e;
@@ -1201,9 +1237,11 @@ hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s) {
 
 size_t spectrum_ws_bytes(int log_n) { return (size_t)3 << log_n << 3; }
 
-hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
-                            const Twiddles& tw, hipStream_t s) {
-    if (log_n > kMaxLogN || log_n > tw.log_t || prime < 0 || prime > 2) return hipErrorInvalidValue;
+// Forward transform of one operand into the 3 residue arrays at res, ending
+// with the row pass in `mode` (kRowFwdOnly or kRowFwdKeep).
+static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int mode, const Twiddles& tw,
+                                  hipStream_t s) {
+    if (log_n > kMaxLogN || log_n > tw.log_t) return hipErrorInvalidValue;
     prepare_kernels();
     const u64 N = 1ull << log_n;
     const Tuning& tu = tuning();
@@ -1211,9 +1249,9 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
     ProductDesc d{};
     d.a = d.b = a;
     d.na = d.nb = na;
-    d.res = (u64*)ws;
+    d.res = res;
     d.log_n = (u32)log_n;
-    d.square = 1;
+    d.kind = kKindSqr;
     for (int pi = 0; pi < pl.n_col; pi++) {
         ColArgs C{};
         C.one = d;
@@ -1243,15 +1281,30 @@ hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out
     R.log_n = log_n;
     R.s0 = pl.row_s;
     R.from_limbs = (pl.n_col == 0);
-    R.mode = kRowFwdOnly;
+    R.mode = mode;
     const int logL = log_n - pl.row_s;
     const int logC = std::max(0, std::min({pl.row_s, tu.row_tile - logL, 6}));
     R.logC = logC;
     const u64 tiles = N >> (logL + logC);
     const size_t lds = ((size_t)1 << (logL + logC)) * sizeof(u64);
     if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
-    FMH_LAUNCH("row_fwd", k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
-    return hipMemcpyAsync(out, d.res + ((u64)prime << log_n), N * sizeof(u64), hipMemcpyDeviceToDevice, s);
+    FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row, dim3((unsigned)tiles, 3, 1),
+               dim3(tu.row_threads), lds, s, R);
+    return hipSuccess;
+}
+
+hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,
+                            const Twiddles& tw, hipStream_t s) {
+    if (prime < 0 || prime > 2) return hipErrorInvalidValue;
+    u64* res = (u64*)ws;
+    hipError_t e = enqueue_forward(a, na, log_n, res, kRowFwdOnly, tw, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(out, res + ((u64)prime << log_n), (sizeof(u64) << log_n), hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t enqueue_resident_spectrum(const u64* a, u64 na, int log_n, u64* spec, const Twiddles& tw,
+                                     hipStream_t s) {
+    return enqueue_forward(a, na, log_n, spec, kRowFwdKeep, tw, s);
 }
 
 }  // namespace fmh

@@ -81,6 +81,87 @@ int fmh_batch_create(const fmh_product* items, u64 count, fmh_batch** batch);
 int fmh_batch_run(fmh_batch* batch, fmh_stream_t stream);
 int fmh_batch_destroy(fmh_batch* batch);
 
+/* ------------------------------------------------- resident spectra
+ * The forward transform of an operand (3 primes x ntt_len words of device
+ * memory, owned by the object) kept for reuse: each product with it runs one
+ * forward transform instead of two, a square of it none.  This is the
+ * semantics of src/fast_mul.c:135-212 conv_single_i (first operand already
+ * transformed, never called in the reference).
+ *
+ * fmh_spectrum_create enqueues the transform of d_in (len limbs) on `stream`;
+ * ntt_len = 0 picks next_pow2(2 len - 1), enough for a square and for products
+ * with operands of up to len limbs.  The operand buffer is not referenced
+ * afterwards.  Products with the spectrum must run after the transform
+ * (same stream, or synchronised) and need len + len2 - 1 <= ntt_len.
+ * Products on different streams may share one spectrum (it is read-only). */
+typedef struct fmh_spectrum fmh_spectrum;
+int fmh_spectrum_create(const u64* d_in, u64 len, u64 ntt_len, fmh_stream_t stream, fmh_spectrum** spec);
+u64 fmh_spectrum_ntt_len(const fmh_spectrum* spec);
+/* d_out (len + len2 limbs) = operand of spec * d_in2 */
+int fmh_mul_spectrum_device(const fmh_spectrum* spec, const u64* d_in2, u64 len2, u64* d_out,
+                            fmh_stream_t stream);
+/* d_out (2 len limbs) = (operand of spec)^2 */
+int fmh_sqr_spectrum_device(const fmh_spectrum* spec, u64* d_out, fmh_stream_t stream);
+/* Waits for the device, then frees the spectrum. */
+int fmh_spectrum_destroy(fmh_spectrum* spec);
+/* The spectrum's device words (3 x ntt_len), operand length and log2(ntt_len). */
+int fmh_spectrum_describe(const fmh_spectrum* spec, const u64** words, u64* len, int* log_n);
+
+/* ------------------------------------------------------------- serving
+ * No counterpart in the reference.  A request loop with continuous batching:
+ * requests (device operands, or a resident spectrum and a device operand)
+ * are submitted as they arrive; a scheduler thread admits each one when its
+ * footprint -- output limbs * 8 + 3 primes * NTT length * 8 B * operands
+ * transformed -- fits in the server's device pool, groups waiting requests of
+ * one size class (transform length, kind) into one batched launch, and keeps
+ * up to `streams` groups in flight on separate streams.  Outputs live in the
+ * pool until fmh_server_release.  Operands must stay valid and unchanged
+ * until their request completes.  Latency is submit -> completion observed
+ * by the scheduler (host clock). */
+typedef struct {
+    u64 pool_bytes;      /* device pool size; 0: 64 GiB */
+    unsigned streams;    /* launch groups in flight; 0: 4 (at most 16) */
+    unsigned max_group;  /* requests per launch group; 0: 1024 */
+    u64 max_group_bytes; /* residue bytes per launch group; 0: pool / 8 */
+} fmh_server_config;
+
+typedef struct {
+    u64 submitted, completed, failed;
+    u64 groups;              /* launch groups run */
+    double mean_group;       /* requests per group */
+    double p50_us, p99_us, max_us, mean_us;  /* submit -> completion */
+    double products_per_s;   /* completed / span_s */
+    double span_s;           /* first submit -> last completion */
+    u64 pool_bytes, pool_peak_bytes;
+} fmh_server_info;
+
+typedef struct fmh_server fmh_server;
+int fmh_server_create(const fmh_server_config* cfg, fmh_server** server);
+/* d_b == NULL: square of d_a.  *id names the request. */
+int fmh_server_submit(fmh_server* server, const u64* d_a, u64 na, const u64* d_b, u64 nb, u64* id);
+/* Product of the spectrum's operand with d_b (nb limbs); d_b == NULL: its square. */
+int fmh_server_submit_spectrum(fmh_server* server, const fmh_spectrum* spec, const u64* d_b, u64 nb,
+                               u64* id);
+/* Ids of requests completed (or failed) since the last poll, at most max. */
+int fmh_server_poll(fmh_server* server, u64* ids, u64 max, u64* n);
+int fmh_server_wait(fmh_server* server, u64 id);
+/* Device pointer and length of a completed request's output (valid until release). */
+int fmh_server_result(fmh_server* server, u64 id, u64** d_out, u64* len);
+int fmh_server_release(fmh_server* server, u64 id);
+/* Waits until every submitted request has completed or failed.  If no work
+ * is in flight and the oldest waiting request cannot be admitted because
+ * unreleased outputs hold the pool, fmh_server_drain and fmh_server_wait
+ * return an error instead of blocking: release completed requests (poll)
+ * while submitting. */
+int fmh_server_drain(fmh_server* server);
+int fmh_server_stats(fmh_server* server, fmh_server_info* stats);
+/* Zeroes the counters and latencies (no request in flight). */
+int fmh_server_reset_stats(fmh_server* server);
+/* Drains, stops the scheduler and frees the pool. */
+int fmh_server_destroy(fmh_server* server);
+/* Message of the calling thread's last failed fmh_server_* call. */
+const char* fmh_server_last_error(void);
+
 /* Largest supported transform length (products with len1 + len2 - 1 up to
  * this many coefficients). */
 u64 fmh_max_ntt_len(void);

@@ -10,5 +10,6 @@ sed 's/extern __shared__ u64 lds\[\];/u64* lds = (u64*)emu_dyn_lds();/' \
     "$ROOT/fast_mul_ntt_amd/csrc/fmh_ntt.hip" > "$OUT/fmh_ntt_emu.cpp"
 g++ -std=c++20 -O1 -g -fsanitize=address -fno-omit-frame-pointer -fPIC -shared -w \
     -I "$ROOT/tests/emu" -I "$ROOT/fast_mul_ntt_amd/csrc" \
-    "$OUT/fmh_ntt_emu.cpp" -x c++ "$ROOT/fast_mul_ntt_amd/csrc/fmh_api.hip" -x none \
+    "$OUT/fmh_ntt_emu.cpp" -x c++ "$ROOT/fast_mul_ntt_amd/csrc/fmh_api.hip" \
+    "$ROOT/fast_mul_ntt_amd/csrc/fmh_serve.hip" -x none \
     "$ROOT/tests/emu/emu_runtime.cpp" -o "$OUT/libfmh_emu.so" -lpthread

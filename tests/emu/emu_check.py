@@ -25,6 +25,22 @@ L.fmh_mul_batch_device.argtypes = [vp, u, vp]
 L.fmh_batch_create.argtypes = [vp, u, ctypes.POINTER(vp)]
 L.fmh_batch_run.argtypes = [vp, vp]
 L.fmh_batch_destroy.argtypes = [vp]
+L.fmh_spectrum_create.argtypes = [vp, u, u, vp, ctypes.POINTER(vp)]
+L.fmh_spectrum_ntt_len.argtypes = [vp]
+L.fmh_spectrum_ntt_len.restype = u
+L.fmh_mul_spectrum_device.argtypes = [vp, vp, u, vp, vp]
+L.fmh_sqr_spectrum_device.argtypes = [vp, vp, vp]
+L.fmh_spectrum_destroy.argtypes = [vp]
+L.fmh_server_create.argtypes = [vp, ctypes.POINTER(vp)]
+L.fmh_server_submit.argtypes = [vp, vp, u, vp, u, ctypes.POINTER(u)]
+L.fmh_server_submit_spectrum.argtypes = [vp, vp, vp, u, ctypes.POINTER(u)]
+L.fmh_server_poll.argtypes = [vp, vp, u, ctypes.POINTER(u)]
+L.fmh_server_wait.argtypes = [vp, u]
+L.fmh_server_result.argtypes = [vp, u, ctypes.POINTER(vp), ctypes.POINTER(u)]
+L.fmh_server_release.argtypes = [vp, u]
+L.fmh_server_drain.argtypes = [vp]
+L.fmh_server_destroy.argtypes = [vp]
+L.fmh_server_last_error.restype = ctypes.c_char_p
 orc = Oracle()
 
 
@@ -75,6 +91,29 @@ for log_n in [0, 3, 13]:
         rc = L.fmh_forward_spectrum_device(prime, limbs.ctypes.data, limbs.size, n, out.ctypes.data, None)
         check(rc == 0 and np.array_equal(out, orc.forward_spectrum(prime, limbs, n)), ("spectrum", log_n, prime))
 
+# resident spectra: default and longer transform lengths, one- and two-pass
+# plans; products with several second operands, the square, a product that
+# does not fit
+for n1, ntt_len, others in [(7, 0, [1, 7]), (7, 2048, [9, 1500]), (600, 0, [500, 600, 1]), (700, 0, [])]:
+    a = orc.splitmix(n1 + 500, n1)
+    h = vp()
+    check(L.fmh_spectrum_create(a.ctypes.data, n1, ntt_len, None, ctypes.byref(h)) == 0, ("spec create", n1))
+    N = L.fmh_spectrum_ntt_len(h)
+    check(N == (ntt_len or 1 << (2 * n1 - 1).bit_length()), ("spec len", n1, N))
+    for n2 in others:
+        b = orc.splitmix(n2 + 600, n2)
+        out = np.full(n1 + n2, 5, np.uint64)
+        check(L.fmh_mul_spectrum_device(h, b.ctypes.data, n2, out.ctypes.data, None) == 0, ("spec mul rc", n1, n2))
+        check(np.array_equal(out, orc.mul(a, b)), ("spec mul", n1, n2))
+    out = np.full(2 * n1, 5, np.uint64)
+    check(L.fmh_sqr_spectrum_device(h, out.ctypes.data, None) == 0, ("spec sqr rc", n1))
+    check(np.array_equal(out, orc.sqr(a)), ("spec sqr", n1))
+    big = np.ones(N, np.uint64)
+    out = np.zeros(N + n1, np.uint64)
+    check(L.fmh_mul_spectrum_device(h, big.ctypes.data, N, out.ctypes.data, None) != 0, ("spec too long", n1))
+    check(L.fmh_spectrum_destroy(h) == 0, ("spec destroy", n1))
+print("spectra ok", flush=True)
+
 # a batch of mixed sizes, squares and an empty operand; then a planned batch replayed
 rng = random.Random(5)
 items, keep = [], []
@@ -106,4 +145,55 @@ for rep in range(2):
         want = orc.sqr(a) if b is None else orc.mul(a, b)
         check(np.array_equal(out[:n], want), ("replay", rep, i))
 check(L.fmh_batch_destroy(h) == 0, "batch destroy")
+
+# request loop: a pool too small to hold every request at once, mixed sizes,
+# squares and spectrum requests; every output equals the oracle's
+class SrvCfg(ctypes.Structure):
+    _fields_ = [("pool_bytes", u), ("streams", ctypes.c_uint), ("max_group", ctypes.c_uint),
+                ("max_group_bytes", u)]
+
+
+cfg = SrvCfg(1 << 20, 3, 8, 0)
+srv = vp()
+check(L.fmh_server_create(ctypes.byref(cfg), ctypes.byref(srv)) == 0, "server create")
+a0 = orc.splitmix(4242, 1000)
+sp = vp()
+check(L.fmh_spectrum_create(a0.ctypes.data, a0.size, 0, None, ctypes.byref(sp)) == 0, "server spec")
+rng = random.Random(11)
+reqs = {}
+for i in range(24):
+    kind = rng.choice(["mul", "mul", "sqr", "spec", "specsqr"])
+    rid = u()
+    if kind in ("mul", "sqr"):
+        n1 = rng.choice([1, 40, 700, 1500])
+        a = orc.splitmix(7000 + i, n1)
+        b = None if kind == "sqr" else orc.splitmix(8000 + i, rng.choice([1, 50, 1200]))
+        rc = L.fmh_server_submit(srv, a.ctypes.data, a.size, None if b is None else b.ctypes.data,
+                                 0 if b is None else b.size, ctypes.byref(rid))
+        want = orc.sqr(a) if b is None else orc.mul(a, b)
+        keep_ops = (a, b)
+    else:
+        b = None if kind == "specsqr" else orc.splitmix(9000 + i, rng.choice([1, 30, 1000]))
+        rc = L.fmh_server_submit_spectrum(srv, sp, None if b is None else b.ctypes.data,
+                                          0 if b is None else b.size, ctypes.byref(rid))
+        want = orc.sqr(a0) if b is None else orc.mul(a0, b)
+        keep_ops = (b,)
+    check(rc == 0, ("submit", i, L.fmh_server_last_error()))
+    reqs[rid.value] = (want, keep_ops)
+check(L.fmh_server_drain(srv) == 0, ("drain", L.fmh_server_last_error()))
+ids = (u * 64)()
+n = u()
+check(L.fmh_server_poll(srv, ids, 64, ctypes.byref(n)) == 0 and n.value == len(reqs), ("poll", n.value))
+for rid, (want, _) in reqs.items():
+    p, ln = vp(), u()
+    check(L.fmh_server_result(srv, rid, ctypes.byref(p), ctypes.byref(ln)) == 0, ("result", rid))
+    got = np.ctypeslib.as_array(ctypes.cast(p, P), shape=(ln.value,)).copy()
+    check(ln.value == want.size and np.array_equal(got, want), ("served product", rid))
+    check(L.fmh_server_release(srv, rid) == 0, ("release", rid))
+too_big = np.ones(1 << 17, np.uint64)
+check(L.fmh_server_submit(srv, too_big.ctypes.data, too_big.size, too_big.ctypes.data, too_big.size,
+                          ctypes.byref(u())) != 0, "footprint above the pool is refused")
+check(L.fmh_server_destroy(srv) == 0, "server destroy")
+check(L.fmh_spectrum_destroy(sp) == 0, "server spec destroy")
+print("server ok", flush=True)
 print("emu ok", flush=True)

@@ -55,6 +55,7 @@ enum {
     hipSuccess = 0,
     hipErrorInvalidValue = 1,
     hipErrorOutOfMemory = 2,
+    hipErrorNotReady = 600,
     hipErrorLaunchFailure = 719,
 };
 typedef struct emu_stream* hipStream_t;
@@ -92,6 +93,7 @@ hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipFuncSetAttribute(const void* f, hipFuncAttribute a, int v);

@@ -202,6 +202,61 @@ def test_batch_graph_replays(fm, orc):
             d.free()
 
 
+@pytest.mark.parametrize("n1,ntt_len,others", [
+    (7, 0, [1, 7]),                       # single row pass
+    (7, 1 << 14, [9, 9000]),              # longer transform than needed, one column pass
+    (3000, 0, [2000, 3000, 1]),           # one column pass
+    (1 << 20, 0, [1 << 20, 12345]),       # NTT 2^21
+])
+def test_resident_spectrum_products(fm, orc, n1, ntt_len, others):
+    """A resident spectrum (reference conv_single_i semantics) reused for
+    several products and a square: each equals the plain product."""
+    a = orc.splitmix(n1 + 500, n1)
+    with fm.Stream() as s, fm.DeviceBuffer.from_host(a, s) as da:
+        with fm.Spectrum(da, n1, ntt_len, s) as sp:
+            assert sp.ntt_len == (ntt_len or 1 << (2 * n1 - 1).bit_length())
+            for n2 in others:
+                b = orc.splitmix(n2 + 600, n2)
+                with fm.DeviceBuffer.from_host(b, s) as db, fm.DeviceBuffer(n1 + n2) as do:
+                    sp.mul(db, n2, do, s)
+                    s.synchronize()
+                    assert np.array_equal(do.download(), orc.mul(a, b)), (n1, n2)
+            with fm.DeviceBuffer(2 * n1) as do:
+                sp.sqr(do, s)
+                s.synchronize()
+                assert np.array_equal(do.download(), orc.sqr(a)), n1
+            # a product longer than the transform is refused, nothing launched
+            with fm.DeviceBuffer(sp.ntt_len) as db, fm.DeviceBuffer(sp.ntt_len + n1) as do:
+                with pytest.raises(fm.FmhError):
+                    sp.mul(db, sp.ntt_len, do, s)
+
+
+def test_resident_spectrum_shared_by_two_streams(fm, orc):
+    """One spectrum read by products on two streams at once."""
+    n1 = 40000
+    a = orc.splitmix(321, n1)
+    bs = [orc.splitmix(322 + i, 30000 + 1000 * i) for i in range(4)]
+    s1, s2 = fm.Stream(), fm.Stream()
+    da = fm.DeviceBuffer.from_host(a, s1)
+    sp = fm.Spectrum(da, n1, 0, s1)
+    s1.synchronize()
+    dbs = [fm.DeviceBuffer.from_host(b) for b in bs]
+    dos = [fm.DeviceBuffer(n1 + b.size) for b in bs]
+    try:
+        for i, (db, do) in enumerate(zip(dbs, dos)):
+            sp.mul(db, db.n, do, s1 if i % 2 else s2)
+        s1.synchronize()
+        s2.synchronize()
+        for b, do in zip(bs, dos):
+            assert np.array_equal(do.download(), orc.mul(a, b)), b.size
+    finally:
+        sp.close()
+        for d in dbs + dos + [da]:
+            d.free()
+        s1.close()
+        s2.close()
+
+
 def _primes61(count, seed):
     import random
     rng = random.Random(seed)

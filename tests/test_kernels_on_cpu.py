@@ -28,7 +28,9 @@ def test_kernels_emulated_on_cpu(tmp_path):
     subprocess.run([os.path.join(ROOT, "tests", "emu", "build_emu.sh"), str(tmp_path)], check=True,
                    capture_output=True, timeout=600)
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0",
-               FMH_TUNE="col_threads=64,row_threads=64")
+               # small tiles: transforms above 256 points take two or more
+               # column passes, so short operands reach every pass schedule
+               FMH_TUNE="col_threads=64,row_threads=64,row_tile=8,row_maxl=7,col_maxl=5")
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "emu_check.py"),
                         str(tmp_path / "libfmh_emu.so")], capture_output=True, text=True, env=env,
                        timeout=1200)
