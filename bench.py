@@ -39,11 +39,11 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against the copy
 # kernel) from rocprofv3 --pmc runs of this workload; profiles/ holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 385935360.0,
-    "row_mul": 152264704.0,
-    "col_inv": 251729920.0,
-    "crt_local": 70279168.0,
-    "carry_apply": 34800640.0,
+    "col_fwd": 202454016.0,
+    "row_mul": 152238080.0,
+    "col_inv": 151102464.0,
+    "crt_local": 70256640.0,
+    "carry_apply": 34788352.0,
 }
 
 

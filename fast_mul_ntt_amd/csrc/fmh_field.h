@@ -9,11 +9,14 @@
 //   p2 = 29 * 2^57 + 1   g = 3
 // so p = PHI * 2^32 + 1 with PHI = p >> 32, and 4p < 2^64.
 //
-// Montgomery reduction (R = 2^64) is done one 32-bit word at a time: for
-// p = 1 (mod 2^32) the per-word quotient digit is simply m = -x_0 mod 2^32,
-// so each word costs one 32x32+64 multiply-add with PHI.  The result is the
-// same integer as the reference's REDC (src/macro.h:109-118): both equal
-// (x + m p) / 2^64 for the unique m < 2^64 with x + m p = 0 (mod 2^64).
+// Montgomery reduction (R = 2^64) is done one 32-bit word at a time in its
+// subtractive form: for p = 1 (mod 2^32), p^-1 = 1 (mod 2^32), so the
+// per-word quotient digit is the word itself and each word costs one
+// 32x32+64 multiply-add with 2^32 - PHI.  redc returns
+//     (x - m p) / 2^64 + p,   m = x p^-1 mod 2^64,
+// which lies in (0, 2p) and is congruent to x 2^-64.  It equals the
+// reference's REDC (src/macro.h:109-118), (x + m' p) / 2^64 with
+// m' = -x p^-1 mod 2^64, except when x = 0 (mod 2^64), where it is p more.
 //
 // Value ranges used by the kernels:
 //   * redc(x, y) < 2p whenever x * y < p * 2^64, because the result is below
@@ -71,25 +74,41 @@ static_assert(Prime<1>::p == (Prime<1>::c << Prime<1>::k) + 1, "p1 form");
 static_assert(Prime<2>::p == (Prime<2>::c << Prime<2>::k) + 1, "p2 form");
 static_assert(Prime<2>::p < (~0ull) / 4, "4p < 2^64 for the largest prime");
 
-// (a * b + m p) / 2^64 with m = -a b p^-1 mod 2^64, by two word steps.
+// Subtractive Montgomery reduction of x = a b in two word steps (see above).
+// With x = hi 2^64 + x1 2^32 + x0:
+//   word 0: m0 = x0;  (x - m0 p) / 2^32 = hi 2^32 + D,  D = x1 - x0 PHI
+//   word 1: m1 = D mod 2^32;  result = hi + floor(D / 2^32) - m1 PHI (+ p)
+// t = x0 (2^32 - PHI) + x1 = D + x0 2^32 is exact in 64 bits, so
+// m1 = lo(t) and e = -floor(D / 2^32) = x0 - hi(t), 0 <= e < 2^30.
 template <int P>
 FMH_HD u64 redc(u64 a, u64 b) {
     constexpr u32 PHI = (u32)(Prime<P>::p >> 32);
+    constexpr u32 NPHI = 0u - PHI;  // 2^32 - PHI
     static_assert(((u64)PHI << 32) + 1 == Prime<P>::p, "p = PHI 2^32 + 1");
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
-    // x = a b = hi 2^64 + x1 2^32 + x0
     const u64 p00 = (u64)a0 * b0;
     const u64 p01 = (u64)a0 * b1 + (p00 >> 32);
     const u64 p10 = (u64)a1 * b0 + (u32)p01;
     const u64 hi = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);
     const u32 x0 = (u32)p00, x1 = (u32)p10;
-    // word 0: m0 = -x0; (x + m0 p) / 2^32 = hi 2^32 + T
-    const u32 m0 = 0u - x0;
-    const u64 T = (u64)m0 * PHI + x1 + (x0 != 0 ? 1u : 0u);
-    // word 1: m1 = -T0; result = hi + T / 2^32 + m1 PHI + [T0 != 0]
-    const u32 y0 = (u32)T;
-    const u32 m1 = 0u - y0;
-    return (u64)m1 * PHI + hi + (T >> 32) + (y0 != 0 ? 1u : 0u);
+    const u64 t = (u64)x0 * NPHI + x1;
+    const u32 m1 = (u32)t;
+    const u32 e = x0 - (u32)(t >> 32);
+    const u64 acc = hi + (Prime<P>::p - e);
+    // result = acc - m1 PHI = (acc - m1 2^32) + m1 (2^32 - PHI)  (mod 2^64)
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Written out so that the subtraction stays one 32-bit op on the high
+    // word and the last step one multiply-add (the compiler otherwise widens
+    // both into 64-bit borrow chains).
+    u32 acc_hi;
+    asm("v_sub_u32 %0, %1, %2" : "=v"(acc_hi) : "v"((u32)(acc >> 32)), "v"(m1));
+    const u64 acc2 = ((u64)acc_hi << 32) | (u32)acc;
+    u64 r, carry;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(m1), "s"(NPHI), "v"(acc2));
+    return r;
+#else
+    return acc - ((u64)m1 << 32) + (u64)m1 * NPHI;
+#endif
 }
 
 // 64 x 64 -> 128-bit product from 32-bit multiply-adds.

@@ -435,18 +435,34 @@ struct ColArgs {
     int nops;  // operands transformed per product: 2 (multiply) or 1
 };
 
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin by
+// workgroup id; each XCD has its own L2.  xcd_slot maps id x of n so that
+// XCD k receives the contiguous slot range [k n / 8, (k + 1) n / 8), in
+// dispatch order: neighbouring slots run on one XCD at about the same time.
+constexpr u32 kXcds = 8;
+__device__ __forceinline__ u32 xcd_slot(u32 x, u32 n) {
+    const u32 xcd = x % kXcds, local = x / kXcds;
+    const u32 per = n / kXcds, rem = n % kXcds;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
 template <int P, bool FWD>
 __device__ void col_body(const ColArgs& A) {
     extern __shared__ u64 lds[];
-    const u32 prod = blockIdx.z / (u32)A.nops;
-    const int op = (int)(blockIdx.z - prod * (u32)A.nops);
+    // slot = tile * nops + operand: the operands of a tile share its twiddle
+    // rows, and neighbouring tiles share 128-byte lines of data and twiddles
+    // (a tile row is 2^logC words), so both stay in one L2.
+    const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
+    const u32 tile = slot / (u32)A.nops;
+    const int op = (int)(slot - tile * (u32)A.nops);
+    const u32 prod = blockIdx.z;
     const ProductDesc& D = product(A.items, A.one, prod);
     const int logL = A.s1 - A.s0;
     const int logB = A.log_n - A.s1;
     const int logC = A.logC;
     const int C = 1 << logC;
     const int n = C << logL;
-    const u64 lam0 = (u64)blockIdx.x << logC;
+    const u64 lam0 = (u64)tile << logC;
     const u64 a = lam0 >> logB;
     const u32 b0 = (u32)(lam0 & ((1ull << logB) - 1));
     const u64 base = (a << (logL + logB)) + b0;
@@ -1135,8 +1151,8 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
         if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, count * nops), dim3(tu.col_threads), lds, s,
-                   C);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * nops), 3, count), dim3(tu.col_threads), lds,
+                   s, C);
     }
     {
         RowArgs R{};
@@ -1192,7 +1208,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     }
     if (j.items.empty()) return hipSuccess;
     if (j.items.size() > 1 && !j.d_items) return hipErrorInvalidValue;
-    constexpr u32 kMaxClass = 32767;  // grid.z holds products x operands
+    constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
     size_t first = 0;
     while (first < j.items.size()) {
         size_t last = first;

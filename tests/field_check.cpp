@@ -19,7 +19,8 @@ static u64 next64() {  // splitmix64
     return z ^ (z >> 31);
 }
 
-// Reference REDC: floor((a b + m p) / 2^64), m = a b (-p^-1) mod 2^64.
+// Reference REDC (src/macro.h:109-118): (a b + m p) / 2^64,
+// m = a b (-p^-1) mod 2^64.
 static u64 redc_ref(u64 a, u64 b, u64 p) {
     u64 inv = p;
     for (int i = 0; i < 6; i++) inv *= 2 - p * inv;
@@ -27,6 +28,13 @@ static u64 redc_ref(u64 a, u64 b, u64 p) {
     const u64 m = (u64)x * (0 - inv);
     const u128 s = x + (u128)m * p;  // < 2^128 for the inputs used below
     return (u64)(s >> 64);
+}
+
+// The kernels' subtractive REDC equals the reference's, plus p when
+// a b = 0 (mod 2^64) (the quotient digit is then 0 in both forms).
+static u64 redc_expected(u64 a, u64 b, u64 p) {
+    const u64 r = redc_ref(a, b, p);
+    return (u64)((u128)a * b) == 0 ? r + p : r;
 }
 
 static int fails = 0;
@@ -45,19 +53,22 @@ static void check_prime(long iters) {
     // redc on every domain the kernels use: (x < 2^64, w < p) and (x, y < 4p with xy < p 2^64)
     for (u64 a : edge)
         for (u64 b : edge) {
-            if (b < p) CHECK(redc<P>(a, b) == redc_ref(a, b, p));
-            if ((u128)a * b < (u128)p << 64) CHECK(redc<P>(a, b) == redc_ref(a, b, p));
+            if (b < p) CHECK(redc<P>(a, b) == redc_expected(a, b, p) && redc<P>(a, b) < p2);
+            if ((u128)a * b < (u128)p << 64) CHECK(redc<P>(a, b) == redc_expected(a, b, p) && redc<P>(a, b) < p2);
         }
     for (long i = 0; i < iters; i++) {
         const u64 a = next64(), w = next64() % p;
         const u64 r = redc<P>(a, w);
-        CHECK(r == redc_ref(a, w, p));
+        CHECK(r == redc_expected(a, w, p));
         CHECK(r < p2);
         const u64 x = next64() % p2, y = next64() % p2;
-        CHECK(redc<P>(x, y) == redc_ref(x, y, p));
+        CHECK(redc<P>(x, y) == redc_expected(x, y, p));
         CHECK(redc<P>(x, y) < p2);
         const u64 z = next64() % p4;
-        CHECK(redc<P>(z, w) == redc_ref(z, w, p) && redc<P>(z, w) < p2);
+        CHECK(redc<P>(z, w) == redc_expected(z, w, p) && redc<P>(z, w) < p2);
+        // low word of the product zero: the one case where the forms differ
+        const u64 za = next64() << 32, zb = (next64() % p) & ~0xFFFFFFFFull;
+        CHECK(redc<P>(za, zb) == redc_ref(za, zb, p) + p && redc<P>(za, zb) < p2);
         // reduce_word = x mod p
         const u64 v = next64();
         CHECK(reduce_word<P>(v) == v % p);
