@@ -22,6 +22,7 @@
  *                    the reference's second library (fft_small abs_mul64)
  *                    compiled from its sources in oracle/_ref
  */
+#define _POSIX_C_SOURCE 200112L /* posix_memalign */
 #include "ntt3_oracle.h"
 
 #include <stdio.h>
@@ -335,6 +336,10 @@ static inline void idit8(const prime_t* P, u64* a) {
 
 static inline unsigned log2u(u64 n) { return n ? 63u - (unsigned)__builtin_clzll(n) : 0u; }
 
+/* Planted mistake in force (tests only; 0 in every normal use). */
+static int g_plant = 0;
+void orc_plant(int kind) { g_plant = kind; }
+
 /* core.h:443-474 dif_i: radix-2^2 passes from rank=len down to 16, then a
  * 4-point tail (even log2) or 8-point tail (odd log2).  The first block of
  * the tail is the "_len" variant that shrinks to 2 / 1 points for len < 4/8. */
@@ -349,6 +354,7 @@ static void dif_core(const prime_t* P, u64* a, const u64* om, u64 len) {
             for (u64 ii = 0; ii < gap; ii++) {
                 u64 t0 = b[ii], t1 = b[gap + ii], t2 = b[2 * gap + ii], t3 = b[3 * gap + ii];
                 u64 o = wl[ii];
+                if (g_plant == 3 && ii + 1 < gap) o = wl[ii + 1];
                 bf_dif2(P, &t0, &t2, w[ii]);
                 bf_dif2(P, &t1, &t3, w[gap + ii]);
                 bf_dif2(P, &t0, &t1, o);
@@ -432,6 +438,11 @@ static int table_alloc(const prime_t* P, table_t* T, u64 ntt_len) {
     T->iom = (u64*)malloc(T->len * sizeof(u64));
     if (!T->om || !T->iom) return -1;
     build_table(P, (int)log2u(T->len), T->om, T->iom);
+    if (g_plant == 2) {
+        u64* t = T->om;
+        T->om = T->iom;
+        T->iom = t;
+    }
     return 0;
 }
 static void table_free(table_t* T) {
@@ -624,6 +635,7 @@ void orc_crt3(u64 a, u64 b, u64 c, u64 out[3]) {
     u64 x[3] = {a, b, c};
     out[0] = out[1] = out[2] = 0;
     for (int i = 0; i < 3; i++) {
+        if (g_plant == 1 && i == 2) continue;
         u64 v = to_int(&PR[i], mm_lazy(&PR[i], x[i], CRT_MINV[i]));
         u64 t[3];
         mul128x64(t, CRT_M[i], v);
@@ -651,7 +663,7 @@ static void crt_carry(u64* const bufs[3], u64 conv_len, u64* out) {
         add192(acc, t);
         out[k] = acc[0];
         acc[0] = acc[1];
-        acc[1] = acc[2];
+        acc[1] = g_plant == 5 ? 0 : acc[2];
         acc[2] = 0;
     }
     out[conv_len] = acc[0];
@@ -685,7 +697,8 @@ void orc_abs_mul64(const u64* in1, u64 len1, const u64* in2, u64 len2, u64* out)
         const prime_t* P = &PR[i];
         table_t T;
         if (table_alloc(P, &T, n)) abort();
-        for (u64 k = 0; k < n; k++) tmp[k] = k < len2 ? to_mont(P, in2[k]) : 0;
+        const u64 pad2 = g_plant == 4 ? (len1 < len2 ? len1 : len2) : len2;
+        for (u64 k = 0; k < n; k++) tmp[k] = k < pad2 ? to_mont(P, in2[k]) : 0;
         conv_rec(P, bufs[i], tmp, bufs[i], &T, n, 1);
         table_free(&T);
     }

@@ -71,6 +71,13 @@ void orc_crt3(u64 a, u64 b, u64 c, u64 out[3]);
 void orc_abs_mul64(const u64* in1, u64 len1, const u64* in2, u64 len2, u64* out);
 void orc_abs_sqr64(const u64* in1, u64 len1, u64* out);
 
+/* Planted mistakes, for the tests that show the oracle's checks catch them
+ * (0 = none): 1 drops a CRT term, 2 swaps the forward and inverse twiddle
+ * tables (wrong sign of the root), 3 shifts a DIF twiddle index by one,
+ * 4 pads the second operand to the first one's length (transposed lengths,
+ * reads kept in bounds), 5 drops the top word of the running carry. */
+void orc_plant(int kind);
+
 /* Independent checks (not part of the reference algorithm). */
 void orc_schoolbook(const u64* a, u64 na, const u64* b, u64 nb, u64* out);
 u64 orc_mod_q(const u64* a, u64 n, u64 q);          /* value mod q, q < 2^63 */

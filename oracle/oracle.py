@@ -60,7 +60,12 @@ class Oracle:
         L.orc_mulmod_q.argtypes = [_u64, _u64, _u64]
         L.orc_mulmod_q.restype = _u64
         L.orc_splitmix64_fill.argtypes = [_u64, _pu64, _u64, ctypes.c_int]
+        L.orc_plant.argtypes = [ctypes.c_int]
         self.L = L
+
+    def plant(self, kind: int) -> None:
+        """Plant a mistake in the oracle (tests of the checks; 0 = none)."""
+        self.L.orc_plant(kind)
 
     def constants(self, prime: int) -> list[int]:
         out = np.zeros(17, dtype=np.uint64)

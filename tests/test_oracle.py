@@ -371,3 +371,45 @@ def test_planted_mistakes_are_caught(orc):
     s = orc.forward_spectrum(0, a[:100], 256)
     s[3] = (int(s[3]) + 1) % PRIMES[0]
     assert not np.array_equal(s, orc.forward_spectrum(0, a[:100], 256))
+
+
+# Plausible mistakes planted in the oracle itself: each one must fail at least
+# one of the checks that pin the oracle (schoolbook products, the O(N^2) DFT
+# of the spectrum, the CRT's defining congruences).
+PLANTS = {1: "dropped CRT term", 2: "root of the wrong sign", 3: "twiddle index off by one",
+          4: "operand lengths transposed", 5: "carry word dropped"}
+
+
+def _checks_that_fail(orc):
+    failed = []
+    a, b = orc.splitmix(501, 300), orc.splitmix(502, 700)
+    if not np.array_equal(orc.mul(a, b), orc.schoolbook(a, b)):
+        failed.append("schoolbook")
+    n, i = 64, 1
+    p = PRIMES[i]
+    limbs = orc.splitmix(503, 40)
+    w = pow(GENERATORS[i], (p - 1) // n, p)
+    xs = [int(v) % p for v in limbs] + [0] * (n - 40)
+    X = [sum(xs[t] * pow(w, t * k, p) for t in range(n)) % p for k in range(n)]
+    if [int(v) for v in orc.forward_spectrum(i, limbs, n)] != [X[bitrev(j, 6)] * R % p for j in range(n)]:
+        failed.append("dft")
+    rng = random.Random(504)
+    for _ in range(50):
+        res = [rng.randrange(p) for p in PRIMES]
+        got = orc.crt3(*res)
+        if any(got % q != r * pow(R, -1, q) % q for r, q in zip(res, PRIMES)):
+            failed.append("crt")
+            break
+    return failed
+
+
+@pytest.mark.parametrize("kind", sorted(PLANTS))
+def test_planted_oracle_mistakes_fail_a_check(orc, kind):
+    assert _checks_that_fail(orc) == []
+    orc.plant(kind)
+    try:
+        failed = _checks_that_fail(orc)
+    finally:
+        orc.plant(0)
+    assert failed, f"planted mistake not caught: {PLANTS[kind]}"
+    assert _checks_that_fail(orc) == []
