@@ -125,11 +125,10 @@ int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
     if (ensure_buffer(w, job_ws_bytes(job), s)) return -1;
     job_bind(job, w.ptr);
     hipError_t e;
-    if (job.items.size() > 1) {
+    const size_t nd = job_desc_bytes(job);
+    if (nd) {
         // descriptors go through pinned memory so the copy can be asynchronous
         Staging& st = g_stage[s];
-        const size_t nd = job.items.size() * sizeof(ProductDesc);
-        const size_t nc = job.chunk_item.size() * sizeof(u32);
         if (st.done) {
             e = hipEventSynchronize(st.done);
             if (e != hipSuccess) return set_err("hipEventSynchronize", e);
@@ -137,19 +136,16 @@ int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
             e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
             if (e != hipSuccess) return set_err("hipEventCreate", e);
         }
-        if (st.bytes < nd + nc) {
+        if (st.bytes < nd) {
             if (st.host) (void)hipHostFree(st.host);
             st.host = nullptr;
             st.bytes = 0;
-            e = hipHostMalloc(&st.host, nd + nc + (nd + nc) / 4, hipHostMallocDefault);
+            e = hipHostMalloc(&st.host, nd + nd / 4, hipHostMallocDefault);
             if (e != hipSuccess) return set_err("hipHostMalloc", e);
-            st.bytes = nd + nc + (nd + nc) / 4;
+            st.bytes = nd + nd / 4;
         }
-        memcpy(st.host, job.items.data(), nd);
-        memcpy((char*)st.host + nd, job.chunk_item.data(), nc);
-        e = hipMemcpyAsync(job.d_items, st.host, nd, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(job.d_chunk_item, (char*)st.host + nd, nc, hipMemcpyHostToDevice, s);
+        job_pack(job, st.host);
+        e = hipMemcpyAsync(job.d_desc, st.host, nd, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(st.done, s);
         if (e != hipSuccess) return set_err("descriptor upload", e);
     }
@@ -409,7 +405,6 @@ int fmh_batch_create(const fmh_product* items, u64 count, fmh_batch** out) {
     if (e != hipSuccess) return fail("hipMalloc", e);
     job_bind(b->job, b->ws);
     e = job_upload(b->job, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail("descriptor upload", e);
     // capture the kernels only (no copies, no allocation, no per-kernel events)
     const bool timing = timing_enabled();

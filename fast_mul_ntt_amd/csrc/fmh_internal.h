@@ -74,6 +74,8 @@ enum ProductKind : u32 {
 };
 
 // One product as launched: device pointers and its share of the workspace.
+// The same struct describes a transform (job items) and a CRT + carry unit
+// (job crt entries); for an unsplit product both are the same values.
 struct ProductDesc {
     const u64* a;
     const u64* b;     // equals a for a square
@@ -85,6 +87,13 @@ struct ProductDesc {
     u32 kind;
     u32 chunk0;       // first carry chunk of this product in the job's chunk arrays
     u32 nchunks;
+    // Split product (CRT entries only): the long operand was cut into
+    // split_n pieces of split_m limbs, each multiplied at 2^log_n points;
+    // piece j's residues are at res + 3 j 2^log_n, and coefficient k of the
+    // product sums those of pieces k / split_m and k / split_m - 1.
+    u64 split_m;
+    u32 split_n;      // 0: not split
+    u32 pad_;
     CrtConsts crt;
 };
 
@@ -102,14 +111,30 @@ struct ProductIn {
     int spec_log_n = 0;
 };
 
+// A spectrum computed inside a job: the short operand of a split product,
+// transformed once at 2^log_n points and read by every piece.
+struct SpecStep {
+    const u64* b;
+    u64 nb;
+    int log_n;
+    u64 off;             // words into the residue region
+    u64* words = nullptr;  // set by job_bind
+};
+
 // A set of products launched together: one product, or a batch of products
-// of any sizes.  Products are grouped into size classes (kind, log_n)
-// and each class runs as one launch per pass; the CRT and carry kernels cover
-// all products at once.
+// of any sizes.  Transforms are grouped into size classes (kind, log_n) and
+// each class runs as one launch per pass; the CRT and carry kernels cover all
+// products at once.  An unbalanced product may be split: its long operand is
+// cut into pieces that each multiply a spectrum of the short one (one
+// transform item per piece, one CRT entry for the whole product).
 struct Job {
-    std::vector<ProductDesc> items;  // sorted by (kind, log_n)
+    std::vector<ProductDesc> items;  // transforms, sorted by (kind, log_n)
+    std::vector<ProductDesc> crt;    // CRT + carry units, request order
+    std::vector<SpecStep> specs;
     std::vector<u64> res_off;        // residue offset of each item (words)
-    std::vector<u32> chunk_item;     // product index of each carry chunk
+    std::vector<u32> item_owner;     // crt entry of each item
+    std::vector<u64> crt_res_off;    // residue offset of each crt entry (words)
+    std::vector<u32> chunk_item;     // crt entry of each carry chunk
     std::vector<std::pair<u64*, u64>> zeros;  // outputs of products with an empty operand
     u64 res_words = 0;
     u32 chunks = 0;
@@ -118,7 +143,9 @@ struct Job {
     u8* fmap = nullptr;
     u32* chunk_map = nullptr;
     u32* chunk_cin = nullptr;
-    ProductDesc* d_items = nullptr;  // device copy of items (batches only)
+    void* d_desc = nullptr;          // device copy of the descriptors (job_pack layout)
+    ProductDesc* d_items = nullptr;  // inside d_desc; null unless uploaded
+    ProductDesc* d_crt = nullptr;
     u32* d_chunk_item = nullptr;
 };
 
@@ -130,7 +157,13 @@ void prepare_kernels();
 int job_build(Job& j, const ProductIn* in, size_t count);
 size_t job_ws_bytes(const Job& j);
 void job_bind(Job& j, void* ws);
-// Copies the descriptors of a batch to the device (nothing for one product).
+// Descriptors that kernels read from device memory (jobs of more than one
+// transform or product): job_desc_bytes() bytes, to be copied from a
+// job_pack()ed host buffer to j.d_desc before job_launch.  0 bytes: nothing
+// to copy (the single descriptor travels as a kernel argument).
+size_t job_desc_bytes(const Job& j);
+void job_pack(const Job& j, void* host);
+// Copies the descriptors with a synchronous-source hipMemcpyAsync.
 hipError_t job_upload(const Job& j, hipStream_t s);
 // Enqueues the kernels (no copies, no allocation: capturable in a graph).
 hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s);

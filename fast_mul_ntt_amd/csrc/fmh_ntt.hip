@@ -31,6 +31,7 @@
 // N, is folded into the CRT constants.
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 #include <mutex>
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
 struct CarryArgs {
     const ProductDesc* items;
     ProductDesc one;
-    const u32* chunk_item;  // product of each chunk (nullptr: product 0)
+    const u32* chunk_item;  // crt entry of each chunk (nullptr: entry 0)
     u8* fmap;               // one map per limb, kCarryChunk per chunk
     u32* chunk_map;         // composed map of each chunk
     u32* chunk_cin;         // carry into each chunk
@@ -694,13 +695,36 @@ __device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
 __device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (2 * c)) & 3u; }
 constexpr u32 kMapId = 0xE4u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
 
-// Residues of coefficient k (zero beyond the convolution).
+// Residues of coefficient k (zero beyond the convolution).  For a split
+// product, coefficient k sums those of piece i = k / m at offset k - i m and
+// of piece i - 1 at offset k - i m + m, where that offset is inside its
+// cyclic length (the sum stays below p0 p1 p2, so adding residues is exact).
+// Inverse transforms leave residues below 4p; each is reduced below 2p so
+// that the sum of two fits 64 bits.
 __device__ __forceinline__ void crt_load(const ProductDesc& D, u64 k, u64& v0, u64& v1, u64& v2) {
     v0 = v1 = v2 = 0;
-    if (k < D.na + D.nb - 1) {
+    if (k >= D.na + D.nb - 1) return;
+    const u64 N = 1ull << D.log_n;
+    if (!D.split_n) {
         v0 = D.res[k];
-        v1 = D.res[k + (1ull << D.log_n)];
-        v2 = D.res[k + (2ull << D.log_n)];
+        v1 = D.res[k + N];
+        v2 = D.res[k + 2 * N];
+        return;
+    }
+    const u32 m = (u32)D.split_m;
+    const u32 i = (u32)k / m;  // k < 2^28
+    const u64 r = k - (u64)i * m;
+    if (i < D.split_n) {
+        const u64* q = D.res + (u64)i * 3 * N + r;
+        v0 = norm2p<0>(q[0]);
+        v1 = norm2p<1>(q[N]);
+        v2 = norm2p<2>(q[2 * N]);
+    }
+    if (i >= 1 && i - 1 < D.split_n && r + m < N) {
+        const u64* q = D.res + (u64)(i - 1) * 3 * N + r + m;
+        v0 += norm2p<0>(q[0]);
+        v1 += norm2p<1>(q[N]);
+        v2 += norm2p<2>(q[2 * N]);
     }
 }
 
@@ -1027,8 +1051,37 @@ void prepare_kernels() {
     });
 }
 
+// Split plan of an unbalanced product n1 x n2 (n1 >= n2, balanced length
+// 2^L): pieces of m limbs of the long operand, each multiplied at 2^k points
+// (m + n2 - 1 <= 2^k) with one spectrum of the short operand.  Cost model:
+// transforms of N points cost N (log N + 4); the split runs 2 K + 1 of them at
+// 2^k, the balanced product 3 at 2^L.  Returns k, or 0 for no split.
+constexpr u32 kMaxPieces = 4096;
+constexpr int kMinPieceLog = 10;
+static int split_plan(u64 n1, u64 n2, int L, u64* m_out, u32* pieces) {
+    if (L <= tuning().row_tile) return 0;  // one row pass already
+    const double balanced = 3.0 * (double)(1ull << L) * (L + 4);
+    double best = 0.75 * balanced;  // split only for a clear gain
+    int best_k = 0;
+    for (int k = std::max(kMinPieceLog, ceil_log2(2 * n2)); k < L; k++) {
+        const u64 m = (1ull << k) - n2 + 1;
+        const u64 K = (n1 + m - 1) / m;
+        if (K < 2 || K > kMaxPieces) continue;
+        const double cost = (2.0 * (double)K + 1.0) * (double)(1ull << k) * (k + 4);
+        if (cost < best) {
+            best = cost;
+            best_k = k;
+            *m_out = m;
+            *pieces = (u32)K;
+        }
+    }
+    return best_k;
+}
+
 int job_build(Job& j, const ProductIn* in, size_t count) {
     j = Job();
+    std::vector<ProductDesc> items;
+    std::vector<u32> owner, part;  // crt entry and piece of each transform item
     for (size_t i = 0; i < count; i++) {
         const ProductIn& p = in[i];
         ProductDesc d{};
@@ -1061,37 +1114,90 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
             if (log_n > p.spec_log_n) return -2;
             log_n = p.spec_log_n;
         }
-        d.log_n = (u32)log_n;
         d.nchunks = (u32)((out_len + kCarryChunk - 1) / kCarryChunk);
-        d.crt = crt_table(log_n);
-        j.items.push_back(d);
+        const u32 ci = (u32)j.crt.size();
+        u64 m = 0;
+        u32 K = 0;
+        const int k = d.kind == kKindMul ? split_plan(std::max(d.na, d.nb), std::min(d.na, d.nb), log_n, &m, &K) : 0;
+        if (k) {
+            // long operand a, short operand b
+            const u64* la = d.na >= d.nb ? d.a : d.b;
+            const u64* sb = d.na >= d.nb ? d.b : d.a;
+            const u64 ln = std::max(d.na, d.nb), sn = std::min(d.na, d.nb);
+            j.specs.push_back(SpecStep{sb, sn, k, 0, nullptr});
+            for (u32 q = 0; q < K; q++) {
+                ProductDesc c{};
+                c.kind = kKindMulSpec;
+                c.a = la + (u64)q * m;
+                c.na = std::min<u64>(m, ln - (u64)q * m);
+                c.nb = sn;
+                c.log_n = (u32)k;
+                c.crt = crt_table(k);
+                items.push_back(c);
+                owner.push_back(ci);
+                part.push_back(q);
+            }
+            d.log_n = (u32)k;
+            d.split_m = m;
+            d.split_n = K;
+            d.crt = crt_table(k);
+        } else {
+            d.log_n = (u32)log_n;
+            d.crt = crt_table(log_n);
+            items.push_back(d);
+            owner.push_back(ci);
+            part.push_back(0);
+        }
+        j.crt.push_back(d);
     }
-    std::stable_sort(j.items.begin(), j.items.end(), [](const ProductDesc& x, const ProductDesc& y) {
-        return x.kind != y.kind ? x.kind < y.kind : x.log_n < y.log_n;
+    // transforms sorted by class; the pieces of one product stay adjacent
+    std::vector<u32> order(items.size());
+    for (u32 i = 0; i < (u32)order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](u32 x, u32 y) {
+        return items[x].kind != items[y].kind ? items[x].kind < items[y].kind : items[x].log_n < items[y].log_n;
     });
-    j.res_off.resize(j.items.size());
-    for (size_t i = 0; i < j.items.size(); i++) {
-        ProductDesc& d = j.items[i];
+    j.crt_res_off.assign(j.crt.size(), 0);
+    j.items.resize(items.size());
+    j.res_off.resize(items.size());
+    j.item_owner.resize(items.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        const ProductDesc& d = items[order[i]];
+        j.items[i] = d;
         j.res_off[i] = j.res_words;
+        j.item_owner[i] = owner[order[i]];
+        if (part[order[i]] == 0) j.crt_res_off[owner[order[i]]] = j.res_words;
         j.res_words += (u64)(d.kind == kKindMul ? 6 : 3) << d.log_n;
+        j.max_log_n = std::max(j.max_log_n, (int)d.log_n);
+    }
+    // spectra of split products, after the residues
+    size_t sp = 0;
+    for (size_t c = 0; c < j.crt.size(); c++) {
+        ProductDesc& d = j.crt[c];
         d.chunk0 = j.chunks;
         j.chunks += d.nchunks;
-        for (u32 c = 0; c < d.nchunks; c++) j.chunk_item.push_back((u32)i);
-        j.max_log_n = std::max(j.max_log_n, (int)d.log_n);
+        for (u32 q = 0; q < d.nchunks; q++) j.chunk_item.push_back((u32)c);
+        if (d.split_n) {
+            SpecStep& st = j.specs[sp++];
+            st.off = j.res_words;
+            j.res_words += (u64)3 << st.log_n;
+        }
     }
     return 0;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+size_t job_desc_bytes(const Job& j) {
+    if (j.items.size() <= 1 && j.crt.size() <= 1) return 0;
+    return align256(j.items.size() * sizeof(ProductDesc)) + align256(j.crt.size() * sizeof(ProductDesc)) +
+           align256(j.chunk_item.size() * sizeof(u32));
+}
+
 size_t job_ws_bytes(const Job& j) {
     size_t b = align256(j.res_words * sizeof(u64));
     b += align256((size_t)j.chunks * kCarryChunk);
     b += 2 * align256((size_t)j.chunks * sizeof(u32));
-    if (j.items.size() > 1) {
-        b += align256(j.items.size() * sizeof(ProductDesc));
-        b += align256((size_t)j.chunks * sizeof(u32));
-    }
+    b += job_desc_bytes(j);
     return b;
 }
 
@@ -1099,30 +1205,56 @@ void job_bind(Job& j, void* ws) {
     char* p = (char*)ws;
     u64* res = (u64*)p;
     p += align256(j.res_words * sizeof(u64));
-    for (size_t i = 0; i < j.items.size(); i++) j.items[i].res = res + j.res_off[i];
+    // residues; the pieces of a split product read its spectrum
+    std::vector<const u64*> spec_of(j.crt.size(), nullptr);
+    size_t sp = 0;
+    for (auto& st : j.specs) st.words = res + st.off;
+    for (size_t c = 0; c < j.crt.size(); c++) {
+        j.crt[c].res = res + j.crt_res_off[c];
+        if (j.crt[c].split_n) spec_of[c] = j.specs[sp++].words;
+    }
+    for (size_t i = 0; i < j.items.size(); i++) {
+        j.items[i].res = res + j.res_off[i];
+        if (spec_of[j.item_owner[i]]) j.items[i].spec = spec_of[j.item_owner[i]];
+    }
     j.fmap = (u8*)p;
     p += align256((size_t)j.chunks * kCarryChunk);
     j.chunk_map = (u32*)p;
     p += align256((size_t)j.chunks * sizeof(u32));
     j.chunk_cin = (u32*)p;
     p += align256((size_t)j.chunks * sizeof(u32));
-    if (j.items.size() > 1) {
+    if (job_desc_bytes(j)) {
+        j.d_desc = p;
         j.d_items = (ProductDesc*)p;
         p += align256(j.items.size() * sizeof(ProductDesc));
+        j.d_crt = (ProductDesc*)p;
+        p += align256(j.crt.size() * sizeof(ProductDesc));
         j.d_chunk_item = (u32*)p;
     } else {
+        j.d_desc = nullptr;
         j.d_items = nullptr;
+        j.d_crt = nullptr;
         j.d_chunk_item = nullptr;
     }
 }
 
+void job_pack(const Job& j, void* host) {
+    char* p = (char*)host;
+    memcpy(p, j.items.data(), j.items.size() * sizeof(ProductDesc));
+    p += align256(j.items.size() * sizeof(ProductDesc));
+    memcpy(p, j.crt.data(), j.crt.size() * sizeof(ProductDesc));
+    p += align256(j.crt.size() * sizeof(ProductDesc));
+    memcpy(p, j.chunk_item.data(), j.chunk_item.size() * sizeof(u32));
+}
+
 hipError_t job_upload(const Job& j, hipStream_t s) {
-    if (j.items.size() <= 1) return hipSuccess;
-    hipError_t e = hipMemcpyAsync(j.d_items, j.items.data(), j.items.size() * sizeof(ProductDesc),
-                                  hipMemcpyHostToDevice, s);
+    const size_t n = job_desc_bytes(j);
+    if (!n) return hipSuccess;
+    std::vector<char> host(n);
+    job_pack(j, host.data());
+    hipError_t e = hipMemcpyAsync(j.d_desc, host.data(), n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
-    return hipMemcpyAsync(j.d_chunk_item, j.chunk_item.data(), j.chunk_item.size() * sizeof(u32),
-                          hipMemcpyHostToDevice, s);
+    return hipStreamSynchronize(s);  // the host copy goes out of scope
 }
 
 // Transforms of one size class: items [first, first + count) share log_n and
@@ -1134,7 +1266,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     const u32 kind = d0.kind;
     const u64 N = 1ull << log_n;
     const Plan pl = make_plan(log_n);
-    const ProductDesc* items = j.d_items ? j.d_items + first : nullptr;
+    const ProductDesc* items = j.items.size() > 1 ? j.d_items + first : nullptr;
     const int nops = kind == kKindMul ? 2 : kind == kKindSqrSpec ? 0 : 1;  // operands to transform
     for (int pi = 0; nops > 0 && pi < pl.n_col; pi++) {
         ColArgs C{};
@@ -1199,6 +1331,9 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     return hipSuccess;
 }
 
+static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int mode, const Twiddles& tw,
+                                  hipStream_t s);
+
 hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     prepare_kernels();
     if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
@@ -1207,7 +1342,12 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     if (j.items.empty()) return hipSuccess;
-    if (j.items.size() > 1 && !j.d_items) return hipErrorInvalidValue;
+    if (job_desc_bytes(j) && !j.d_desc) return hipErrorInvalidValue;
+    // spectra of the short operands of split products
+    for (const SpecStep& st : j.specs) {
+        hipError_t e = enqueue_forward(st.b, st.nb, st.log_n, st.words, kRowFwdKeep, tw, s);
+        if (e != hipSuccess) return e;
+    }
     constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
     size_t first = 0;
     while (first < j.items.size()) {
@@ -1220,14 +1360,14 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
         first = last;
     }
     CarryArgs K{};
-    K.items = j.d_items;
-    K.one = j.items[0];
-    K.chunk_item = j.d_chunk_item;
+    K.items = j.crt.size() > 1 ? j.d_crt : nullptr;
+    K.one = j.crt[0];
+    K.chunk_item = j.crt.size() > 1 ? j.d_chunk_item : nullptr;
     K.fmap = j.fmap;
     K.chunk_map = j.chunk_map;
     K.chunk_cin = j.chunk_cin;
     FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kThreads), 0, s, K);
-    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.items.size()), dim3(kThreads), 0, s, K);
+    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.crt.size()), dim3(kThreads), 0, s, K);
     FMH_LAUNCH("carry_apply", k_carry_apply, dim3(j.chunks), dim3(kThreads), 0, s, K);
     return hipSuccess;
 }

@@ -264,24 +264,19 @@ struct fmh_server {
     void launch(int si, const Job& job, const std::vector<Request*>& group, u64 ws_off, u64 ws) {
         Slot& s = slots[si];
         hipError_t e = hipSuccess;
-        if (job.items.size() > 1) {
-            const size_t nd = job.items.size() * sizeof(ProductDesc);
-            const size_t nc = job.chunk_item.size() * sizeof(u32);
-            if (s.staging_bytes < nd + nc) {
+        const size_t nd = job_desc_bytes(job);
+        if (nd) {
+            if (s.staging_bytes < nd) {
                 if (s.staging) (void)hipHostFree(s.staging);
                 s.staging = nullptr;
                 s.staging_bytes = 0;
-                e = hipHostMalloc(&s.staging, nd + nc, hipHostMallocDefault);
-                if (e == hipSuccess) s.staging_bytes = nd + nc;
+                e = hipHostMalloc(&s.staging, nd, hipHostMallocDefault);
+                if (e == hipSuccess) s.staging_bytes = nd;
             }
             if (e == hipSuccess) {
                 // the slot is idle, so its previous upload has completed
-                memcpy(s.staging, job.items.data(), nd);
-                memcpy((char*)s.staging + nd, job.chunk_item.data(), nc);
-                e = hipMemcpyAsync(job.d_items, s.staging, nd, hipMemcpyHostToDevice, s.stream);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(job.d_chunk_item, (char*)s.staging + nd, nc, hipMemcpyHostToDevice,
-                                       s.stream);
+                job_pack(job, s.staging);
+                e = hipMemcpyAsync(job.d_desc, s.staging, nd, hipMemcpyHostToDevice, s.stream);
             }
         }
         if (e == hipSuccess) e = job_launch(job, tw, s.stream);
@@ -337,7 +332,6 @@ namespace {
 const char* const kStalled =
     "the pool is full of unreleased outputs: release completed requests before waiting";
 
-u64 residue_bytes(u32 kind, int log_n) { return (u64)(kind == kKindMul ? 6 : 3) * sizeof(u64) << log_n; }
 
 int submit(fmh_server* s, const ProductIn& in, u64* id) {
     if (!s || !id) return fail("NULL argument");
@@ -351,11 +345,11 @@ int submit(fmh_server* s, const ProductIn& in, u64* id) {
         delete r;
         return fail("requests need non-empty operands");
     }
-    const ProductDesc& d = probe.items[0];
+    const ProductDesc& d = probe.crt[0];
     r->out_len = d.na + d.nb;
-    r->log_n = (int)d.log_n;
-    r->kind = d.kind;
-    r->ws_bytes = residue_bytes(d.kind, r->log_n);
+    r->log_n = (int)probe.items[0].log_n;  // size class: the transforms'
+    r->kind = probe.items[0].kind;
+    r->ws_bytes = job_ws_bytes(probe);
     // alone in a group it needs its output and the job's workspace
     if (round_up(r->out_len * sizeof(u64)) + round_up(job_ws_bytes(probe)) > s->pool.capacity()) {
         delete r;

@@ -79,6 +79,12 @@ for n1, n2 in [(1, 1), (1, 5), (5, 1), (3, 17), (64, 64), (255, 257), (1, 2500),
 for n in [1, 2, 33, 2049]:
     a = orc.splitmix(n + 3, n)
     check(np.array_equal(sqr(a), orc.sqr(a)), ("sqr", n))
+# unbalanced products: the long operand is cut into pieces (several piece
+# counts, a last piece shorter than the others, either operand the long one)
+for n1, n2 in [(3000, 100), (100, 3000), (5000, 7), (2600, 300)]:
+    a, b = orc.splitmix(n1 + 17, n1), orc.splitmix(n2 + 19, n2)
+    check(np.array_equal(mul(a, b), orc.mul(a, b)), ("unbalanced", n1, n2))
+print("unbalanced ok", flush=True)
 ones = np.full(300, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
 check(np.array_equal(mul(ones, ones[:200]), orc.mul(ones, ones[:200])), "all ones")
 
@@ -120,7 +126,7 @@ items, keep = [], []
 for i in range(14):
     n1 = rng.choice([0, 1, 7, 64, 300, 2000]) if i != 0 else 0
     sq = rng.random() < 0.3
-    n2 = n1 if sq else rng.choice([1, 9, 100, 3000])
+    n2 = n1 if sq else rng.choice([1, 9, 100, 3000, 5])
     a = orc.splitmix(900 + i, n1)
     b = None if sq else orc.splitmix(950 + i, n2)
     out = np.full(max(1, n1 + n2), 7, np.uint64)

@@ -49,6 +49,17 @@ def test_products_vs_oracle(fm, orc, n1, n2):
     assert np.array_equal(got, want)
 
 
+# Unbalanced shapes, where the long operand is cut into pieces that share one
+# spectrum of the short operand: several piece counts, a short last piece,
+# either operand the long one.
+@pytest.mark.parametrize("n1,n2", [(1 << 20, 3000), (786435, 70000), (70000, 1000), (3000, 300001),
+                                   (1 << 21, 1 << 17), (65536 * 7 + 3, 2)])
+def test_unbalanced_products_vs_oracle(fm, orc, n1, n2):
+    a = orc.splitmix(n1 * 7 + 3, n1)
+    b = orc.splitmix(n2 * 11 + 4, n2)
+    assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
+
+
 @pytest.mark.parametrize("n", [4097, 65536, 1 << 20])
 def test_squares_vs_oracle(fm, orc, n):
     a = orc.splitmix(n + 77, n)
