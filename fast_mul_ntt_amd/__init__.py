@@ -41,6 +41,8 @@ __all__ = [
     "ServerConfig",
     "ServerInfo",
     "Server",
+    "set_tuning",
+    "get_tuning",
     "timing_enable",
     "timing_collect",
     "FmhError",
@@ -144,6 +146,8 @@ def lib() -> ctypes.CDLL:
     L.fmh_server_destroy.argtypes = [_vp]
     L.fmh_server_last_error.restype = ctypes.c_char_p
     L.fmh_last_error.restype = ctypes.c_char_p
+    L.fmh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p]
+    L.fmh_get_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p, _u64]
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
     L.fmh_timing_only.argtypes = [ctypes.c_char_p]
     L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
@@ -484,6 +488,17 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def set_tuning(log_n: int, spec: str) -> None:
+    """Pass geometry of 2^log_n-point transforms (-1: all sizes; "" = defaults)."""
+    _check(lib().fmh_set_tuning(log_n, spec.encode()), "fmh_set_tuning")
+
+
+def get_tuning(log_n: int) -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().fmh_get_tuning(log_n, buf, len(buf)), "fmh_get_tuning")
+    return buf.value.decode()
 
 
 def timing_enable(on: bool, only: str | None = None) -> None:

@@ -510,6 +510,19 @@ int fmh_stream_synchronize(fmh_stream_t stream) {
 
 const char* fmh_last_error(void) { return t_err.c_str(); }
 
+int fmh_set_tuning(int log_n, const char* spec) {
+    return set_tuning(log_n, spec) ? set_err("malformed tuning spec or size class") : 0;
+}
+
+int fmh_get_tuning(int log_n, char* buf, u64 buflen) {
+    if (log_n < 0 || log_n > kMaxLogN || !buf || !buflen) return set_err("bad argument");
+    const Tuning t = tuning(log_n);
+    snprintf(buf, buflen,
+             "col_logc=%d,col_maxl=%d,row_maxl=%d,row_tile=%d,col_threads=%d,row_threads=%d,row_radix=%d",
+             t.col_logc, t.col_maxl, t.row_maxl, t.row_tile, t.col_threads, t.row_threads, t.row_radix);
+    return 0;
+}
+
 int fmh_timing_enable(int on) {
     timing_enable(on != 0);
     return 0;

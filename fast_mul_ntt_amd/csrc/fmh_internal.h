@@ -21,9 +21,17 @@ constexpr int kMaxPassThreads = 1024; // launch bound of the NTT pass kernels
 constexpr int kCarryPerThread = 4;    // limbs per thread in the carry scan
 constexpr int kCarryChunk = kThreads * kCarryPerThread;
 
-// Tile geometry of the NTT passes.  The defaults can be overridden for tuning
-// sweeps with FMH_TUNE="col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,
-// col_threads=512,row_threads=512" (read once per process).
+// Tile geometry of the NTT passes, per transform length 2^log_n.  Sources,
+// later ones overriding earlier ones:
+//   * the defaults below;
+//   * a tuning table, one "log_n spec" line per size class, read once from
+//     $FMH_TUNE_TABLE or else tuning_gfx950.txt beside libfmh.so (written by
+//     fast_mul_ntt_amd/autotune.py);
+//   * fmh_set_tuning(log_n, spec) at run time;
+//   * $FMH_TUNE (one spec for every size; for sweeps and tests).
+// A spec reads "col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,
+// col_threads=512,row_threads=512,row_radix=4"; missing keys keep their
+// value.
 struct Tuning {
     int col_logc = 3;       // column passes: 2^col_logc adjacent lines per tile
     int col_maxl = 10;      // column passes: at most 2^col_maxl points per line
@@ -32,8 +40,12 @@ struct Tuning {
                             // transform of at most 2^row_tile points is one pass
     int col_threads = 512;  // threads per workgroup, column passes
     int row_threads = 512;  // threads per workgroup, row pass
+    int row_radix = 4;      // row pass: at most row_radix stages per LDS round (3 or 4)
 };
-const Tuning& tuning();
+Tuning tuning(int log_n);
+// Applies spec to the class log_n (-1: every class); "" restores the
+// defaults of that class.  Returns 0, or -1 for a malformed spec.
+int set_tuning(int log_n, const char* spec);
 
 // One transform plan: stages [0, n) of the in-place radix-2 DIF split into
 // column passes [s_i, s_{i+1}) and a final row pass [s_row, n).
@@ -45,6 +57,7 @@ struct Plan {
 };
 
 Plan make_plan(int log_n);
+Plan make_plan(int log_n, const Tuning& tu);
 
 // Per-call CRT constants (Garner form, see crt_consts in fmh_ntt.hip).
 struct CrtConsts {

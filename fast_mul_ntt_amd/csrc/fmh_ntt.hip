@@ -33,6 +33,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -93,45 +95,103 @@ CrtConsts crt_consts(int log_n) {
     return c;
 }
 
-const Tuning& tuning() {
-    static Tuning t;
-    static std::once_flag once;
-    std::call_once(once, [] {
-        const char* env = getenv("FMH_TUNE");
-        if (!env) return;
-        std::string spec(env);
-        size_t pos = 0;
-        while (pos < spec.size()) {
-            size_t end = spec.find(',', pos);
-            if (end == std::string::npos) end = spec.size();
-            const std::string kv = spec.substr(pos, end - pos);
-            const size_t eq = kv.find('=');
-            if (eq != std::string::npos) {
-                const std::string k = kv.substr(0, eq);
-                const int v = atoi(kv.c_str() + eq + 1);
-                if (k == "col_logc") t.col_logc = v;
-                if (k == "col_maxl") t.col_maxl = v;
-                if (k == "row_maxl") t.row_maxl = v;
-                if (k == "row_tile") t.row_tile = v;
-                if (k == "col_threads") t.col_threads = v;
-                if (k == "row_threads") t.row_threads = v;
-            }
-            pos = end + 1;
-        }
-        // keep the geometry inside what the kernels assume
-        t.col_logc = std::max(0, std::min(t.col_logc, 5));
-        t.col_maxl = std::max(1, std::min(t.col_maxl, 13));
-        t.row_tile = std::max(2, std::min(t.row_tile, 13));
-        t.row_maxl = std::max(t.col_logc, std::min(t.row_maxl, t.row_tile));
-        if (t.col_maxl + t.col_logc > 14) t.col_maxl = 14 - t.col_logc;  // 128 KiB of LDS
-        t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
-        t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
-    });
+// Parses "key=value,..." into t; returns false on a malformed spec.
+static bool parse_tuning(const std::string& spec, Tuning& t) {
+    size_t pos = 0;
+    while (pos < spec.size()) {
+        size_t end = spec.find(',', pos);
+        if (end == std::string::npos) end = spec.size();
+        const std::string kv = spec.substr(pos, end - pos);
+        pos = end + 1;
+        if (kv.empty()) continue;
+        const size_t eq = kv.find('=');
+        if (eq == std::string::npos) return false;
+        const std::string k = kv.substr(0, eq);
+        const int v = atoi(kv.c_str() + eq + 1);
+        if (k == "col_logc") t.col_logc = v;
+        else if (k == "col_maxl") t.col_maxl = v;
+        else if (k == "row_maxl") t.row_maxl = v;
+        else if (k == "row_tile") t.row_tile = v;
+        else if (k == "col_threads") t.col_threads = v;
+        else if (k == "row_threads") t.row_threads = v;
+        else if (k == "row_radix") t.row_radix = v;
+        else return false;
+    }
+    return true;
+}
+
+// Keeps the geometry inside what the kernels assume.
+static Tuning clamp_tuning(Tuning t) {
+    t.col_logc = std::max(0, std::min(t.col_logc, 5));
+    t.col_maxl = std::max(4, std::min(t.col_maxl, 13));  // <= 7 column passes (Plan::col_s)
+    t.row_tile = std::max(2, std::min(t.row_tile, 13));
+    t.row_maxl = std::max(t.col_logc, std::min(t.row_maxl, t.row_tile));
+    if (t.col_maxl + t.col_logc > 14) t.col_maxl = 14 - t.col_logc;  // 128 KiB of LDS
+    t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
+    t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
+    t.row_radix = t.row_radix == 3 ? 3 : 4;
     return t;
 }
 
-Plan make_plan(int log_n) {
-    const Tuning& tu = tuning();
+static std::mutex g_tune_mu;
+static Tuning g_tune[kMaxLogN + 1];
+static bool g_tune_ready = false;
+static std::string g_tune_env;  // $FMH_TUNE, applied last
+
+// Directory of this shared library (for the tuning table beside it).
+static std::string library_dir() {
+    Dl_info info{};
+    if (dladdr((const void*)&library_dir, &info) && info.dli_fname) {
+        std::string f(info.dli_fname);
+        const size_t slash = f.rfind('/');
+        if (slash != std::string::npos) return f.substr(0, slash);
+    }
+    return ".";
+}
+
+static void tuning_init_locked() {
+    if (g_tune_ready) return;
+    g_tune_ready = true;
+    const char* tab = getenv("FMH_TUNE_TABLE");
+    const std::string path = tab ? std::string(tab) : library_dir() + "/tuning_gfx950.txt";
+    if (FILE* f = fopen(path.c_str(), "r")) {
+        char line[512];
+        while (fgets(line, sizeof(line), f)) {
+            int log_n = -1;
+            char spec[480] = {0};
+            if (line[0] == '#' || sscanf(line, "%d %479s", &log_n, spec) != 2) continue;
+            if (log_n < 0 || log_n > kMaxLogN) continue;
+            Tuning t;
+            if (parse_tuning(spec, t)) g_tune[log_n] = t;
+        }
+        fclose(f);
+    }
+    if (const char* env = getenv("FMH_TUNE")) g_tune_env = env;
+}
+
+Tuning tuning(int log_n) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tuning_init_locked();
+    Tuning t = g_tune[std::max(0, std::min(log_n, kMaxLogN))];
+    if (!g_tune_env.empty()) (void)parse_tuning(g_tune_env, t);
+    return clamp_tuning(t);
+}
+
+int set_tuning(int log_n, const char* spec) {
+    if (log_n < -1 || log_n > kMaxLogN) return -1;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tuning_init_locked();
+    for (int l = (log_n < 0 ? 0 : log_n); l <= (log_n < 0 ? kMaxLogN : log_n); l++) {
+        Tuning t = (spec && *spec) ? g_tune[l] : Tuning();
+        if (spec && *spec && !parse_tuning(spec, t)) return -1;
+        g_tune[l] = t;
+    }
+    return 0;
+}
+
+Plan make_plan(int log_n) { return make_plan(log_n, tuning(log_n)); }
+
+Plan make_plan(int log_n, const Tuning& tu) {
     Plan pl{};
     pl.log_n = log_n;
     if (log_n <= tu.row_tile) {
@@ -542,17 +602,161 @@ struct RowLine {
     u32 active;
 };
 
-template <int P>
+// Row-tile LDS layout.  Word i of a tile lives at row_swz(i): the words of
+// each aligned 16-word block are permuted by bits 4-7 of i (low three bits
+// XOR bits 4-6, bit 3 XOR the parity of bits 4-7).  Every access pattern of
+// the row pass then touches 16 distinct 8-byte bank pairs per 16 lanes:
+//   * rounds with half-span hl >= 16 and the global <-> LDS copies read 16
+//     consecutive aligned words (a permutation of one block);
+//   * the fused round reads word 8k + q in lane k (bits 4-6 select the low
+//     bits, bit 3 comes from k);
+//   * rounds with hl = 8 read words u + 8q of two neighbouring groups whose
+//     bits 4-7 differ in exactly one bit, so their bit-3 parities differ.
+__device__ __forceinline__ int row_swz(int i) {
+    const int m = (i >> 4) & 15;
+    return i ^ ((m & 7) | ((__builtin_popcount(m) & 1) << 3));
+}
+
+// One round of R consecutive stages on one operand tile (word c * L + j),
+// half-spans hl 2^(R-1) .. hl for DIF (decreasing) or hl .. hl 2^(R-1) for
+// DIT (increasing), hl = 2^lq.  Group g holds the 2^R words
+// base + q hl of one line; the twiddle of the pair (q, q + hq) at stage
+// half-span H = hq hl is W[H + (j mod H)] = W[(hq + q mod hq) hl + u].
+template <int P, int R, bool DIF>
+__device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, int lq, int g,
+                                                const u64* __restrict__ W) {
+    constexpr int Q = 1 << R;
+    const int hl = 1 << lq;
+    const int u = g & (hl - 1);
+    const int rest = g >> lq;
+    const int lb = logL - lq - R;  // log2 of groups per line
+    const int blk = rest & ((1 << lb) - 1);
+    const int c = rest >> lb;
+    const int base = (c << logL) + (blk << (lq + R)) + u;
+    u64 x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) x[q] = t[row_swz(base + (q << lq))];
+#pragma unroll
+    for (int st = 0; st < R; st++) {
+        const int hq = DIF ? (1 << (R - 1 - st)) : (1 << st);
+        u64 w[Q / 2];
+#pragma unroll
+        for (int r = 0; r < hq; r++) w[r] = W[((hq + r) << lq) + u];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q & hq) continue;
+            if (DIF)
+                bf_dif<P>(x[q], x[q + hq], w[q & (hq - 1)]);
+            else
+                bf_dit<P>(x[q], x[q + hq], w[q & (hq - 1)]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) t[row_swz(base + (q << lq))] = x[q];
+}
+
+// A round over the tiles of nt operands (t1 = second operand or null).
+template <int P, int R, bool DIF>
+__device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W) {
+    const int per = n >> R;
+    for (int g = threadIdx.x; g < nt * per; g += blockDim.x) {
+        if (g < per)
+            row_round_group<P, R, DIF>(t0, logL, lq, g, W);
+        else
+            row_round_group<P, R, DIF>(t1, logL, lq, g - per, W);
+    }
+    __syncthreads();
+}
+
+template <int P, bool DIF, int RMAX>
+__device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W) {
+    switch (r) {
+        case 1: row_round<P, 1, DIF>(t0, t1, nt, n, logL, lq, W); break;
+        case 2: row_round<P, 2, DIF>(t0, t1, nt, n, logL, lq, W); break;
+        case 3: row_round<P, 3, DIF>(t0, t1, nt, n, logL, lq, W); break;
+        default: row_round<P, (RMAX > 3 ? 4 : 3), DIF>(t0, t1, nt, n, logL, lq, W); break;
+    }
+}
+
+// The fused middle of the row pass: for each group of 2^R consecutive words
+// (the last R forward stages, half-spans 2^(R-1) .. 1), finish the forward
+// transform of operand A (and B), form the pointwise product in registers,
+// and run the first R inverse stages, then store A.  Twiddles here do not
+// depend on the lane.  Modes kRowFwdOnly / kRowFwdKeep stop after the
+// forward stages.
+template <int P, int R>
+__device__ void row_fused(const RowArgs& A, const RowLine* s_line, u64* tA, u64* tB, int n, int logL) {
+    constexpr int Q = 1 << R;
+    const u64* __restrict__ W = A.tw[P];
+    const u64* __restrict__ IW = A.itw[P];
+    const bool two = A.mode == kRowMul;
+    const bool fwd = A.mode != kRowSqrSpec;
+    const bool inv = A.mode == kRowMul || A.mode == kRowSqr || A.mode == kRowMulSpec || A.mode == kRowSqrSpec;
+    for (int g = threadIdx.x; g < (n >> R); g += blockDim.x) {
+        const int base = g << R;
+        u64 x[Q], y[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            x[q] = tA[row_swz(base + q)];
+            if (two) y[q] = tB[row_swz(base + q)];
+        }
+        if (fwd) {
+#pragma unroll
+            for (int st = 0; st < R; st++) {
+                const int hq = 1 << (R - 1 - st);
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    if (q & hq) continue;
+                    const u64 w = W[hq + (q & (hq - 1))];
+                    bf_dif<P>(x[q], x[q + hq], w);
+                    if (two) bf_dif<P>(y[q], y[q + hq], w);
+                }
+            }
+        }
+        if (!inv) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) tA[row_swz(base + q)] = x[q];
+            continue;
+        }
+        if (A.mode == kRowMulSpec) {
+            const RowLine& li = s_line[base >> logL];
+            const u64 j0 = li.g0 + (u64)(base & ((1 << logL) - 1));
+#pragma unroll
+            for (int q = 0; q < Q; q++) x[q] = redc<P>(x[q], li.active ? li.rs[j0 + q] : 0ull);
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; q++) x[q] = redc<P>(x[q], two ? y[q] : x[q]);
+        }
+#pragma unroll
+        for (int st = 0; st < R; st++) {
+            const int hq = 1 << st;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (q & hq) continue;
+                bf_dit<P>(x[q], x[q + hq], IW[hq + (q & (hq - 1))]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) tA[row_swz(base + q)] = x[q];
+    }
+    __syncthreads();
+}
+
+// RMAX: the most stages one round takes (4: radix-16 rounds for two
+// operands, more registers; 3: radix-8 rounds, fewer registers and more
+// waves per SIMD).
+template <int P, int RMAX>
 __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     extern __shared__ u64 lds[];
     const int logL = A.log_n - A.s0;
     const int logC = A.logC;
     const int C = 1 << logC;
     const int n = C << logL;
+    const int nl = n < 16 ? 16 : n;  // words per operand tile in LDS (row_swz stays below 16)
     const bool two = (A.mode == kRowMul);
     const bool spec_in = (A.mode == kRowSqrSpec);  // the tile starts as a spectrum
     u64* __restrict__ tA = lds;
-    u64* __restrict__ tB = lds + n;
+    u64* __restrict__ tB = lds + nl;
     if ((int)threadIdx.x < C) {
         const u64 lambda = ((u64)blockIdx.x << logC) + threadIdx.x;
         const u64 prod = lambda >> A.s0;
@@ -589,8 +793,8 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
                 if (two) vb = li.rb[g];
             }
         }
-        tA[e] = va;
-        if (two) tB[e] = vb;
+        tA[row_swz(e)] = va;
+        if (two) tB[row_swz(e)] = vb;
     };
     if (by_line) {
         for (int c = 0; c < C; c++) {
@@ -604,54 +808,63 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
         }
     }
     __syncthreads();
-    if (two)
-        tile_dif<P, 1, 2>(tA, tB, logL, logC, 0, 0u, A.tw[P]);
-    else if (!spec_in)
-        tile_dif<P, 1, 1>(tA, nullptr, logL, logC, 0, 0u, A.tw[P]);
+    // Stage schedule: the last RF forward stages, the pointwise product and
+    // the first RF inverse stages run fused in registers; the other stages
+    // run as rounds of up to 4 stages (both operands) or 3 (one operand).
+    const int RF = logL < 3 ? logL : 3;
+    const int nt = two ? 2 : 1;
+    if (!spec_in) {
+        const int rmax = two ? RMAX : 3;
+        int top = logL;
+        while (top > RF) {
+            const int left = top - RF;
+            const int r = left % rmax ? left % rmax : rmax;
+            row_round_r<P, true, RMAX>(r, tA, tB, nt, n, logL, top - r, A.tw[P]);
+            top -= r;
+        }
+    }
+    switch (RF) {
+        case 0: row_fused<P, 0>(A, s_line, tA, tB, n, logL); break;
+        case 1: row_fused<P, 1>(A, s_line, tA, tB, n, logL); break;
+        case 2: row_fused<P, 2>(A, s_line, tA, tB, n, logL); break;
+        default: row_fused<P, 3>(A, s_line, tA, tB, n, logL); break;
+    }
     if (A.mode == kRowFwdOnly || A.mode == kRowFwdKeep) {
         const u64 r2 = A.r2[P];
         const bool mont = A.mode == kRowFwdOnly;
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
             const RowLine& li = s_line[e >> logL];
-            if (li.active)
-                li.ra[li.g0 + (e & ((1 << logL) - 1))] = canon<P>(mont ? redc<P>(tA[e], r2) : norm2p<P>(tA[e]));
+            const u64 v = tA[row_swz(e)];
+            if (li.active) li.ra[li.g0 + (e & (L - 1))] = canon<P>(mont ? redc<P>(v, r2) : norm2p<P>(v));
         }
         return;
     }
-    if (A.mode == kRowMulSpec) {
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-            const RowLine& li = s_line[e >> logL];
-            const u64 y = li.active ? li.rs[li.g0 + (e & ((1 << logL) - 1))] : 0ull;
-            tA[e] = redc<P>(tA[e], y);
-        }
-    } else {
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-            const u64 x = tA[e];
-            tA[e] = redc<P>(x, two ? tB[e] : x);
-        }
+    for (int lq = RF; lq < logL;) {
+        const int r = logL - lq < 3 ? logL - lq : 3;
+        row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P]);
+        lq += r;
     }
-    __syncthreads();
-    tile_dit<P, 1>(tA, logL, logC, 0, 0u, A.itw[P]);
     if (by_line) {
         for (int c = 0; c < C; c++) {
             const RowLine li = s_line[c];
             if (!li.active) continue;
-            for (int j = threadIdx.x; j < L; j += blockDim.x) li.ra[li.g0 + j] = tA[(c << logL) + j];
+            for (int j = threadIdx.x; j < L; j += blockDim.x) li.ra[li.g0 + j] = tA[row_swz((c << logL) + j)];
         }
     } else {
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
             const RowLine& li = s_line[e >> logL];
-            if (li.active) li.ra[li.g0 + (e & (L - 1))] = tA[e];
+            if (li.active) li.ra[li.g0 + (e & (L - 1))] = tA[row_swz(e)];
         }
     }
 }
 
+template <int RMAX>
 __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
     __shared__ RowLine s_line[kMaxRowLines];
     switch (blockIdx.y) {
-        case 0: row_body<0>(A, s_line); break;
-        case 1: row_body<1>(A, s_line); break;
-        default: row_body<2>(A, s_line); break;
+        case 0: row_body<0, RMAX>(A, s_line); break;
+        case 1: row_body<1, RMAX>(A, s_line); break;
+        default: row_body<2, RMAX>(A, s_line); break;
     }
 }
 
@@ -845,25 +1058,38 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     if (tid == 0) A.chunk_map[tile] = agg;
 }
 
-// One workgroup per product: the carry into each of its chunks.  Thread t
-// owns a contiguous segment of the product's chunks.
+// One workgroup per product: the carry into each of its chunks.  The chunk
+// maps pass through LDS in tiles of kScanTile (coalesced loads); within a
+// tile, thread t owns a contiguous segment, and the carry out of each tile
+// enters the next.
+constexpr int kScanTile = 4096;
+
 __global__ __launch_bounds__(kThreads) void k_chunk_scan(CarryArgs A) {
     __shared__ u32 s_wave[kThreads / 64];
+    __shared__ u32 s_m[kScanTile];
     const int tid = threadIdx.x;
     const ProductDesc& D = product(A.items, A.one, blockIdx.x);
     const u32 n = D.nchunks;
     const u32* maps = A.chunk_map + D.chunk0;
     u32* cin = A.chunk_cin + D.chunk0;
-    const u32 per = (n + blockDim.x - 1) / blockDim.x;
-    const u32 beg = min(n, (u32)tid * per), end = min(n, beg + per);
-    u32 seg = kMapId;
-    for (u32 i = beg; i < end; i++) seg = map_compose(maps[i], seg);
-    u32 all;
-    const u32 excl = block_excl_scan_maps(seg, s_wave, &all);
-    u32 c = map_apply(excl, 0);
-    for (u32 i = beg; i < end; i++) {
-        cin[i] = c;
-        c = map_apply(maps[i], c);
+    u32 carry = 0;  // carry into the current tile
+    for (u32 base = 0; base < n; base += kScanTile) {
+        const u32 cnt = min(n - base, (u32)kScanTile);
+        for (u32 i = tid; i < cnt; i += blockDim.x) s_m[i] = maps[base + i];
+        __syncthreads();
+        const u32 per = (cnt + blockDim.x - 1) / blockDim.x;
+        const u32 beg = min(cnt, (u32)tid * per), end = min(cnt, beg + per);
+        u32 seg = kMapId;
+        for (u32 i = beg; i < end; i++) seg = map_compose(s_m[i], seg);
+        u32 all;
+        const u32 excl = block_excl_scan_maps(seg, s_wave, &all);
+        u32 c = map_apply(excl, carry);
+        for (u32 i = beg; i < end; i++) {
+            cin[base + i] = c;
+            c = map_apply(s_m[i], c);
+        }
+        carry = map_apply(all, carry);
+        __syncthreads();  // s_m and s_wave are reused by the next tile
     }
 }
 
@@ -1027,7 +1253,7 @@ static const CrtConsts& crt_table(int log_n) {
 // 160 KiB per workgroup, static LDS included; kernels default to 64 KiB
 // unless the limit is raised.  Launches that would exceed it are refused
 // with an error instead of being sent.
-static size_t g_max_dyn[3] = {0, 0, 0};  // k_col<true>, k_col<false>, k_row
+static size_t g_max_dyn[4] = {0, 0, 0, 0};  // k_col<true>, k_col<false>, k_row<4>, k_row<3>
 
 static size_t raise_lds_limit(const void* f) {
     constexpr size_t kLdsPerGroup = 160 * 1024, kDefault = 64 * 1024;
@@ -1047,7 +1273,8 @@ void prepare_kernels() {
     std::call_once(once, [] {
         g_max_dyn[0] = raise_lds_limit((const void*)k_col<true>);
         g_max_dyn[1] = raise_lds_limit((const void*)k_col<false>);
-        g_max_dyn[2] = raise_lds_limit((const void*)k_row);
+        g_max_dyn[2] = raise_lds_limit((const void*)k_row<4>);
+        g_max_dyn[3] = raise_lds_limit((const void*)k_row<3>);
     });
 }
 
@@ -1059,7 +1286,7 @@ void prepare_kernels() {
 constexpr u32 kMaxPieces = 4096;
 constexpr int kMinPieceLog = 10;
 static int split_plan(u64 n1, u64 n2, int L, u64* m_out, u32* pieces) {
-    if (L <= tuning().row_tile) return 0;  // one row pass already
+    if (L <= tuning(L).row_tile) return 0;  // one row pass already
     const double balanced = 3.0 * (double)(1ull << L) * (L + 4);
     double best = 0.75 * balanced;  // split only for a clear gain
     int best_k = 0;
@@ -1260,12 +1487,12 @@ hipError_t job_upload(const Job& j, hipStream_t s) {
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
-    const Tuning& tu = tuning();
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
+    const Tuning tu = tuning(log_n);
     const u32 kind = d0.kind;
     const u64 N = 1ull << log_n;
-    const Plan pl = make_plan(log_n);
+    const Plan pl = make_plan(log_n, tu);
     const ProductDesc* items = j.items.size() > 1 ? j.d_items + first : nullptr;
     const int nops = kind == kKindMul ? 2 : kind == kKindSqrSpec ? 0 : 1;  // operands to transform
     for (int pi = 0; nops > 0 && pi < pl.n_col; pi++) {
@@ -1307,9 +1534,13 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         while (logC > 0 && (1ull << (logC - 1)) >= lines) logC--;
         R.logC = logC;
         const u64 tiles = (lines + (1ull << logC) - 1) >> logC;
-        const size_t lds = (size_t)(kind == kKindMul ? 2 : 1) * ((size_t)1 << (logL + logC)) * sizeof(u64);
-        if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
-        FMH_LAUNCH(kRowName[kind], k_row, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
+        const size_t lds = (size_t)(kind == kKindMul ? 2 : 1) * std::max<size_t>(16, (size_t)1 << (logL + logC)) *
+                           sizeof(u64);  // row tiles take at least 16 words (row_swz)
+        if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
+        if (tu.row_radix == 3)
+            FMH_LAUNCH(kRowName[kind], k_row<3>, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
+        else
+            FMH_LAUNCH(kRowName[kind], k_row<4>, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
     }
     for (int pi = pl.n_col - 1; pi >= 0; pi--) {
         ColArgs C{};
@@ -1400,8 +1631,8 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     if (log_n > kMaxLogN || log_n > tw.log_t) return hipErrorInvalidValue;
     prepare_kernels();
     const u64 N = 1ull << log_n;
-    const Tuning& tu = tuning();
-    const Plan pl = make_plan(log_n);
+    const Tuning tu = tuning(log_n);
+    const Plan pl = make_plan(log_n, tu);
     ProductDesc d{};
     d.a = d.b = a;
     d.na = d.nb = na;
@@ -1442,10 +1673,14 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     const int logC = std::max(0, std::min({pl.row_s, tu.row_tile - logL, 6}));
     R.logC = logC;
     const u64 tiles = N >> (logL + logC);
-    const size_t lds = ((size_t)1 << (logL + logC)) * sizeof(u64);
-    if (lds > g_max_dyn[2]) return hipErrorInvalidValue;
-    FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row, dim3((unsigned)tiles, 3, 1),
-               dim3(tu.row_threads), lds, s, R);
+    const size_t lds = std::max<size_t>(16, (size_t)1 << (logL + logC)) * sizeof(u64);
+    if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
+    if (tu.row_radix == 3)
+        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<3>, dim3((unsigned)tiles, 3, 1),
+                   dim3(tu.row_threads), lds, s, R);
+    else
+        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<4>, dim3((unsigned)tiles, 3, 1),
+                   dim3(tu.row_threads), lds, s, R);
     return hipSuccess;
 }
 

@@ -182,6 +182,15 @@ int fmh_stream_synchronize(fmh_stream_t stream);
 int fmh_copy_device(u64* d_dst, const u64* d_src, u64 n, fmh_stream_t stream);
 const char* fmh_last_error(void);
 
+/* Pass geometry of transforms of 2^log_n points (log_n = -1: every size):
+ * spec "col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,col_threads=512,
+ * row_threads=512" (missing keys keep their value; "" restores the
+ * defaults).  Defaults come from tuning_gfx950.txt beside the library
+ * (written by fast_mul_ntt_amd/autotune.py) or $FMH_TUNE_TABLE; $FMH_TUNE
+ * overrides every size.  Results never depend on the geometry. */
+int fmh_set_tuning(int log_n, const char* spec);
+int fmh_get_tuning(int log_n, char* buf, u64 buflen);
+
 /* Per-kernel timing with HIP events recorded on each kernel's launch stream
  * (for benchmarks).  fmh_timing_collect writes "name count total_ms" lines
  * for the launches recorded since the last collect and returns the number of

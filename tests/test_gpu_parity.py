@@ -60,6 +60,31 @@ def test_unbalanced_products_vs_oracle(fm, orc, n1, n2):
     assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
 
 
+def test_every_tuning_candidate_gives_the_same_product(fm, orc):
+    """Each geometry the autotuner may choose (one parameter changed at a
+    time, and two extreme combinations), on transforms of 2^14 and 2^17
+    points: products equal the oracle's."""
+    from fast_mul_ntt_amd.autotune import CANDIDATES
+    specs = [f"{k}={v}" for k, vs in CANDIDATES.items() for v in vs]
+    specs += ["row_maxl=8,row_tile=10,col_maxl=7,col_logc=4",
+              "row_tile=13,row_maxl=12,col_logc=2,col_threads=1024,row_threads=1024"]
+    cases = []
+    for lg in (14, 17):
+        n = 1 << (lg - 1)
+        a, b = orc.splitmix(lg, n), orc.splitmix(lg + 100, n - 3)
+        cases.append((lg, a, b, orc.mul(a, b)))
+    saved = {lg: fm.get_tuning(lg) for lg, *_ in cases}
+    try:
+        for spec in specs:
+            for lg, a, b, want in cases:
+                fm.set_tuning(lg, "")
+                fm.set_tuning(lg, spec)
+                assert np.array_equal(fm.abs_mul64(a, b), want), (spec, lg)
+    finally:
+        for lg, t in saved.items():
+            fm.set_tuning(lg, t)
+
+
 @pytest.mark.parametrize("n", [4097, 65536, 1 << 20])
 def test_squares_vs_oracle(fm, orc, n):
     a = orc.splitmix(n + 77, n)
