@@ -110,6 +110,27 @@ def cpu_baseline(a: np.ndarray, b: np.ndarray, budget_s: float = 12.0):
     return prod, times
 
 
+def cpu_fft_small(a: np.ndarray, b: np.ndarray, want: np.ndarray, budget_s: float = 8.0):
+    """Time the reference's second library, fft_small (FLINT small-prime FFT,
+    built from the reference's sources into oracle/_ref), on the same operands,
+    single thread; None where it was not built."""
+    try:
+        from oracle.oracle import RefFftSmall
+        ref = RefFftSmall()
+    except (ImportError, OSError):
+        return None
+    got = ref.mul(a, b)  # first call: fft_small builds its context
+    if not np.array_equal(got, want):
+        raise AssertionError("fft_small product differs from the oracle's")
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 20):
+        t0 = time.perf_counter()
+        ref.mul(a, b)
+        times.append(time.perf_counter() - t0)
+    return times
+
+
 def log_uniform_lengths(rng: np.random.Generator, count: int, lo: int, hi: int) -> np.ndarray:
     return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(np.int64)
 
@@ -370,8 +391,8 @@ def run_sizes(args, fm) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")
@@ -530,6 +551,17 @@ def main() -> int:
             result["cpu_baseline"]["cpu"] = model
         except Exception:
             pass
+        fs = cpu_fft_small(a, b, want)
+        if fs is not None:
+            result["cpu_fft_small"] = {
+                "value": round(float(np.mean(fs)) * 1e6, 1),
+                "median": round(float(np.median(fs)) * 1e6, 1),
+                "unit": "µs",
+                "cores": 1,
+                "kind": "reference",
+                "sample": f"{len(fs)} x the same 1048576x1048576-limb abs_mul64 of the reference's "
+                          "fft_small (built from its sources, 1 thread, after one warm-up call)",
+            }
     print(json.dumps(result))
     return 0
 

@@ -120,6 +120,7 @@ def lib() -> ctypes.CDLL:
     L.fmh_stream_create.argtypes = [ctypes.POINTER(_vp)]
     L.fmh_stream_destroy.argtypes = [_vp]
     L.fmh_stream_synchronize.argtypes = [_vp]
+    L.fmh_trim.argtypes = []
     L.fmh_copy_device.argtypes = [_vp, _vp, _u64, _vp]
     L.fmh_mul_batch_device.argtypes = [ctypes.POINTER(FmhProduct), _u64, _vp]
     L.fmh_batch_create.argtypes = [ctypes.POINTER(FmhProduct), _u64, ctypes.POINTER(_vp)]
@@ -146,8 +147,8 @@ def lib() -> ctypes.CDLL:
     L.fmh_server_destroy.argtypes = [_vp]
     L.fmh_server_last_error.restype = ctypes.c_char_p
     L.fmh_last_error.restype = ctypes.c_char_p
-    L.fmh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p]
-    L.fmh_get_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p, _u64]
+    L.fmh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.fmh_get_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, _u64]
     L.fmh_timing_enable.argtypes = [ctypes.c_int]
     L.fmh_timing_only.argtypes = [ctypes.c_char_p]
     L.fmh_timing_collect.argtypes = [ctypes.c_char_p, _u64]
@@ -490,14 +491,15 @@ class Server:
         self.close()
 
 
-def set_tuning(log_n: int, spec: str) -> None:
-    """Pass geometry of 2^log_n-point transforms (-1: all sizes; "" = defaults)."""
-    _check(lib().fmh_set_tuning(log_n, spec.encode()), "fmh_set_tuning")
+def set_tuning(log_n: int, spec: str, batched: bool = False) -> None:
+    """Pass geometry of 2^log_n-point transforms (-1: all sizes; "" = defaults),
+    for single products or for batched launches of the class."""
+    _check(lib().fmh_set_tuning(log_n, spec.encode(), 1 if batched else 0), "fmh_set_tuning")
 
 
-def get_tuning(log_n: int) -> str:
+def get_tuning(log_n: int, batched: bool = False) -> str:
     buf = ctypes.create_string_buffer(256)
-    _check(lib().fmh_get_tuning(log_n, buf, len(buf)), "fmh_get_tuning")
+    _check(lib().fmh_get_tuning(log_n, 1 if batched else 0, buf, len(buf)), "fmh_get_tuning")
     return buf.value.decode()
 
 

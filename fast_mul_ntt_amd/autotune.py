@@ -4,8 +4,11 @@
 
 For each size class 2^L it times a balanced product of two 2^(L-1)-limb
 operands (device-resident, seeded operands) under candidate geometries and
-keeps the fastest.  The search is coordinate descent over the parameters of a
-geometry spec, starting from the current entry:
+keeps the fastest; for classes up to 2^17 it also tunes the geometry of
+batched launches, timing one batch of many such products (the table's
+"b L spec" lines, used when a launch carries several products of a class).
+The search is coordinate descent over the parameters of a geometry spec,
+starting from the defaults:
 
     row_maxl     points per row-pass line          2^8 .. 2^12
     row_tile     points per row-pass tile          2^10 .. 2^13
@@ -15,10 +18,13 @@ geometry spec, starting from the current entry:
     row_threads  threads per row workgroup         256 .. 1024
     row_radix    stages per row-pass LDS round     3 or 4
 
-Every candidate's product is compared word for word with the product under
-the default geometry, so a geometry that computed a different result would
-stop the run.  Geometries the kernels cannot launch (too much LDS) are
-skipped.  The table is written as "L spec" lines; libfmh.so reads it from
+A candidate replaces the incumbent only if it is at least 3% faster in
+interleaved measurements (candidate and incumbent timed alternately, the
+minimum of several bursts each), which keeps clock drift on the device from
+choosing geometries.  Every candidate's product is compared word for word
+with the product under the default geometry, so a geometry that computed a
+different result would stop the run.  Geometries the kernels cannot launch
+(too much LDS) are skipped.  libfmh.so reads the table from
 tuning_gfx950.txt beside itself (or $FMH_TUNE_TABLE) at start-up.
 """
 
@@ -68,58 +74,79 @@ def fmt(d: dict[str, int]) -> str:
 
 
 class Case:
-    """A balanced 2^(L-1) x 2^(L-1) multiply, resident on the device."""
+    """Balanced 2^(L-1) x 2^(L-1) multiplies resident on the device: one
+    product, or (batched) a batch of `count` of them in one launch."""
 
-    def __init__(self, L: int, stream: fm.Stream):
+    def __init__(self, L: int, stream: fm.Stream, count: int = 1):
         self.L = L
         self.n = 1 << (L - 1)
         self.s = stream
-        self.a = fm.DeviceBuffer.from_host(splitmix64(1000 + L, self.n), stream)
-        self.b = fm.DeviceBuffer.from_host(splitmix64(2000 + L, self.n), stream)
-        self.out = fm.DeviceBuffer(2 * self.n)
+        self.batched = count > 1
+        self.bufs = []
+        self.items = []
+        for i in range(count):
+            a = fm.DeviceBuffer.from_host(splitmix64(1000 * L + 2 * i, self.n), stream)
+            b = fm.DeviceBuffer.from_host(splitmix64(1000 * L + 2 * i + 1, self.n), stream)
+            out = fm.DeviceBuffer(2 * self.n)
+            self.bufs += [a, b, out]
+            self.items.append((a, self.n, b, self.n, out))
         self.want = None
+        self.per = 1
 
     def run_once(self) -> None:
-        fm.mul_device(self.a, self.n, self.b, self.n, self.out, self.s)
+        if self.batched:
+            fm.mul_batch_device(self.items, self.s)
+        else:
+            a, n, b, _, out = self.items[0]
+            fm.mul_device(a, n, b, n, out, self.s)
 
-    def time(self, spec: str, reps: int) -> float | None:
-        """Median µs of one product under spec (None: not launchable)."""
-        fm.set_tuning(self.L, spec)
+    def check(self, spec: str) -> bool:
+        """Runs once under spec; False if the geometry cannot launch."""
+        fm.set_tuning(self.L, spec, self.batched)
         try:
             self.run_once()
             self.s.synchronize()
         except fm.FmhError:
-            return None
-        got = self.out.download()
+            return False
+        got = np.concatenate([it[4].download() for it in self.items[:4]])
         if self.want is None:
             self.want = got
         elif not np.array_equal(got, self.want):
             raise AssertionError(f"2^{self.L}: geometry {spec} changed the product")
-        per = max(1, int(2e-3 / max(self._last or 1e-4, 1e-6)))
-        samples = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            for _ in range(per):
-                self.run_once()
-            self.s.synchronize()
-            samples.append((time.perf_counter() - t0) / per)
-        self._last = float(np.median(samples))
-        return self._last * 1e6
+        return True
 
-    _last = None
+    def burst(self, spec: str) -> float:
+        """Seconds per launch over one burst of `per` launches under spec."""
+        fm.set_tuning(self.L, spec, self.batched)
+        t0 = time.perf_counter()
+        for _ in range(self.per):
+            self.run_once()
+        self.s.synchronize()
+        return (time.perf_counter() - t0) / self.per
+
+    def compare(self, inc: str, cand: str, reps: int) -> tuple[float, float]:
+        """Interleaved minima (µs) of the incumbent and a candidate."""
+        ti, tc = [], []
+        for _ in range(reps):
+            ti.append(self.burst(inc))
+            tc.append(self.burst(cand))
+        return min(ti) * 1e6, min(tc) * 1e6
 
     def free(self) -> None:
-        for d in (self.a, self.b, self.out):
+        for d in self.bufs:
             d.free()
 
 
-def tune_size(L: int, stream: fm.Stream, reps: int, rounds: int) -> tuple[str, float, float]:
-    case = Case(L, stream)
+def tune(L: int, stream: fm.Stream, reps: int, rounds: int, count: int = 1) -> tuple[str, float, float]:
+    case = Case(L, stream, count)
     try:
-        fm.set_tuning(L, "")
-        base = parse(fm.get_tuning(L))
-        default_us = case.time(fmt(base), reps)  # also the reference product
-        best, best_us = dict(base), default_us
+        fm.set_tuning(L, "", case.batched)
+        base = parse(fm.get_tuning(L, case.batched))
+        case.check(fmt(base))  # the reference product
+        t = case.burst(fmt(base))
+        case.per = max(1, int(3e-3 / t))
+        default_us, _ = case.compare(fmt(base), fmt(base), reps)
+        best = dict(base)
         for _ in range(rounds):
             improved = False
             for key, values in CANDIDATES.items():
@@ -128,13 +155,16 @@ def tune_size(L: int, stream: fm.Stream, reps: int, rounds: int) -> tuple[str, f
                         continue
                     cand = dict(best)
                     cand[key] = v
-                    us = case.time(fmt(cand), reps)
-                    if us is not None and us < best_us * 0.99:
-                        best, best_us, improved = cand, us, True
+                    if not case.check(fmt(cand)):
+                        continue
+                    inc_us, cand_us = case.compare(fmt(best), fmt(cand), reps)
+                    if cand_us < 0.97 * inc_us:
+                        best, improved = cand, True
             if not improved:
                 break
-        fm.set_tuning(L, fmt(best))
-        return fmt(best), best_us, default_us
+        best_us, def_us = case.compare(fmt(best), fmt(base), reps)
+        fm.set_tuning(L, fmt(best), case.batched)
+        return fmt(best), best_us, def_us
     finally:
         case.free()
 
@@ -143,6 +173,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--min-log", type=int, default=10)
     ap.add_argument("--max-log", type=int, default=25)
+    ap.add_argument("--max-batch-log", type=int, default=17)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=DEFAULT_TABLE)
@@ -151,10 +182,16 @@ def main() -> int:
         raise fm.FmhError(fm.lib().fmh_last_error().decode())
     stream = fm.Stream()
     lines = ["# NTT pass geometry per transform length 2^L (written by fast_mul_ntt_amd/autotune.py)",
-             "# L spec  # us tuned / us default, balanced 2^(L-1) x 2^(L-1) multiply"]
+             "# L spec      # us tuned / us default, one balanced 2^(L-1) x 2^(L-1) multiply",
+             "# b L spec    # the same for one batched launch of max(8, 2^22 / 2^L) such products"]
     for L in range(args.min_log, args.max_log + 1):
-        spec, us, d_us = tune_size(L, stream, args.reps, args.rounds)
+        spec, us, d_us = tune(L, stream, args.reps, args.rounds)
         lines.append(f"{L} {spec}  # {us:.1f} / {d_us:.1f}")
+        print(lines[-1], flush=True)
+    for L in range(args.min_log, args.max_batch_log + 1):
+        count = max(8, (1 << 22) >> L)
+        spec, us, d_us = tune(L, stream, args.reps, args.rounds, count)
+        lines.append(f"b {L} {spec}  # {us:.1f} / {d_us:.1f}")
         print(lines[-1], flush=True)
     stream.close()
     with open(args.out, "w") as f:

@@ -503,6 +503,25 @@ int fmh_stream_destroy(fmh_stream_t stream) {
     return e == hipSuccess ? 0 : set_err("hipStreamDestroy", e);
 }
 
+int fmh_trim(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_err("hipGetDevice", e);
+    if (dev < 0 || dev >= kMaxDevices) return set_err("device index out of range");
+    DevCtx& c = g_ctx[dev];
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return set_err("hipDeviceSynchronize", e);
+    for (auto& kv : c.ws)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    c.ws.clear();
+    for (Buffer* b : {&c.stage_a, &c.stage_b, &c.stage_out}) {
+        if (b->ptr) (void)hipFree(b->ptr);
+        *b = Buffer();
+    }
+    return 0;
+}
+
 int fmh_stream_synchronize(fmh_stream_t stream) {
     hipError_t e = hipStreamSynchronize((hipStream_t)stream);
     return e == hipSuccess ? 0 : set_err("hipStreamSynchronize", e);
@@ -510,13 +529,13 @@ int fmh_stream_synchronize(fmh_stream_t stream) {
 
 const char* fmh_last_error(void) { return t_err.c_str(); }
 
-int fmh_set_tuning(int log_n, const char* spec) {
-    return set_tuning(log_n, spec) ? set_err("malformed tuning spec or size class") : 0;
+int fmh_set_tuning(int log_n, const char* spec, int batched) {
+    return set_tuning(log_n, spec, batched != 0) ? set_err("malformed tuning spec or size class") : 0;
 }
 
-int fmh_get_tuning(int log_n, char* buf, u64 buflen) {
+int fmh_get_tuning(int log_n, int batched, char* buf, u64 buflen) {
     if (log_n < 0 || log_n > kMaxLogN || !buf || !buflen) return set_err("bad argument");
-    const Tuning t = tuning(log_n);
+    const Tuning t = tuning(log_n, batched != 0);
     snprintf(buf, buflen,
              "col_logc=%d,col_maxl=%d,row_maxl=%d,row_tile=%d,col_threads=%d,row_threads=%d,row_radix=%d",
              t.col_logc, t.col_maxl, t.row_maxl, t.row_tile, t.col_threads, t.row_threads, t.row_radix);

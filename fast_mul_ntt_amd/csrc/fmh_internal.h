@@ -24,7 +24,8 @@ constexpr int kCarryChunk = kThreads * kCarryPerThread;
 // Tile geometry of the NTT passes, per transform length 2^log_n.  Sources,
 // later ones overriding earlier ones:
 //   * the defaults below;
-//   * a tuning table, one "log_n spec" line per size class, read once from
+//   * a tuning table, one "log_n spec" line per size class (and "b log_n
+//     spec" lines for batched launches), read once from
 //     $FMH_TUNE_TABLE or else tuning_gfx950.txt beside libfmh.so (written by
 //     fast_mul_ntt_amd/autotune.py);
 //   * fmh_set_tuning(log_n, spec) at run time;
@@ -42,10 +43,15 @@ struct Tuning {
     int row_threads = 512;  // threads per workgroup, row pass
     int row_radix = 4;      // row pass: at most row_radix stages per LDS round (3 or 4)
 };
-Tuning tuning(int log_n);
+// batched: the geometry for a launch that carries several products of the
+// class (table lines "b <log_n> <spec>"); single products use the plain
+// entries.
+Tuning tuning(int log_n, bool batched = false);
 // Applies spec to the class log_n (-1: every class); "" restores the
 // defaults of that class.  Returns 0, or -1 for a malformed spec.
-int set_tuning(int log_n, const char* spec);
+int set_tuning(int log_n, const char* spec, bool batched = false);
+// Launches of at least this many products of one class use the batched table.
+constexpr unsigned kBatchedClass = 4;
 
 // One transform plan: stages [0, n) of the in-place radix-2 DIF split into
 // column passes [s_i, s_{i+1}) and a final row pass [s_row, n).
@@ -84,6 +90,8 @@ enum ProductKind : u32 {
     kKindSqr = 1,     // a * a: one forward transform
     kKindMulSpec = 2, // a * (operand of spec): a transformed, spec read
     kKindSqrSpec = 3, // (operand of spec)^2: no forward transform
+    kKindSpecFwd = 4, // forward transform of a into a spectrum at res (the
+                      // short operand of a split product); launched first
 };
 
 // One product as launched: device pointers and its share of the workspace.
@@ -124,28 +132,19 @@ struct ProductIn {
     int spec_log_n = 0;
 };
 
-// A spectrum computed inside a job: the short operand of a split product,
-// transformed once at 2^log_n points and read by every piece.
-struct SpecStep {
-    const u64* b;
-    u64 nb;
-    int log_n;
-    u64 off;             // words into the residue region
-    u64* words = nullptr;  // set by job_bind
-};
-
 // A set of products launched together: one product, or a batch of products
 // of any sizes.  Transforms are grouped into size classes (kind, log_n) and
 // each class runs as one launch per pass; the CRT and carry kernels cover all
 // products at once.  An unbalanced product may be split: its long operand is
-// cut into pieces that each multiply a spectrum of the short one (one
-// transform item per piece, one CRT entry for the whole product).
+// cut into pieces that each multiply a spectrum of the short one (a
+// kKindSpecFwd item computes the spectrum, one kKindMulSpec item per piece
+// reads it, one CRT entry covers the whole product).
 struct Job {
     std::vector<ProductDesc> items;  // transforms, sorted by (kind, log_n)
     std::vector<ProductDesc> crt;    // CRT + carry units, request order
-    std::vector<SpecStep> specs;
     std::vector<u64> res_off;        // residue offset of each item (words)
     std::vector<u32> item_owner;     // crt entry of each item
+    std::vector<u8> item_spec;       // 1: the item computes a split product's spectrum
     std::vector<u64> crt_res_off;    // residue offset of each crt entry (words)
     std::vector<u32> chunk_item;     // crt entry of each carry chunk
     std::vector<std::pair<u64*, u64>> zeros;  // outputs of products with an empty operand

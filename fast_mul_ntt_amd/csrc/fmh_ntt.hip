@@ -134,7 +134,7 @@ static Tuning clamp_tuning(Tuning t) {
 }
 
 static std::mutex g_tune_mu;
-static Tuning g_tune[kMaxLogN + 1];
+static Tuning g_tune[2][kMaxLogN + 1];  // [batched][log_n]
 static bool g_tune_ready = false;
 static std::string g_tune_env;  // $FMH_TUNE, applied last
 
@@ -159,32 +159,34 @@ static void tuning_init_locked() {
         while (fgets(line, sizeof(line), f)) {
             int log_n = -1;
             char spec[480] = {0};
-            if (line[0] == '#' || sscanf(line, "%d %479s", &log_n, spec) != 2) continue;
+            const int b = line[0] == 'b' ? 1 : 0;
+            if (line[0] == '#' || sscanf(line + b, "%d %479s", &log_n, spec) != 2) continue;
             if (log_n < 0 || log_n > kMaxLogN) continue;
             Tuning t;
-            if (parse_tuning(spec, t)) g_tune[log_n] = t;
+            if (parse_tuning(spec, t)) g_tune[b][log_n] = t;
         }
         fclose(f);
     }
     if (const char* env = getenv("FMH_TUNE")) g_tune_env = env;
 }
 
-Tuning tuning(int log_n) {
+Tuning tuning(int log_n, bool batched) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tuning_init_locked();
-    Tuning t = g_tune[std::max(0, std::min(log_n, kMaxLogN))];
+    Tuning t = g_tune[batched ? 1 : 0][std::max(0, std::min(log_n, kMaxLogN))];
     if (!g_tune_env.empty()) (void)parse_tuning(g_tune_env, t);
     return clamp_tuning(t);
 }
 
-int set_tuning(int log_n, const char* spec) {
+int set_tuning(int log_n, const char* spec, bool batched) {
     if (log_n < -1 || log_n > kMaxLogN) return -1;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tuning_init_locked();
+    Tuning* tab = g_tune[batched ? 1 : 0];
     for (int l = (log_n < 0 ? 0 : log_n); l <= (log_n < 0 ? kMaxLogN : log_n); l++) {
-        Tuning t = (spec && *spec) ? g_tune[l] : Tuning();
+        Tuning t = (spec && *spec) ? tab[l] : Tuning();
         if (spec && *spec && !parse_tuning(spec, t)) return -1;
-        g_tune[l] = t;
+        tab[l] = t;
     }
     return 0;
 }
@@ -1284,6 +1286,7 @@ void prepare_kernels() {
 // transforms of N points cost N (log N + 4); the split runs 2 K + 1 of them at
 // 2^k, the balanced product 3 at 2^L.  Returns k, or 0 for no split.
 constexpr u32 kMaxPieces = 4096;
+constexpr u32 kSpecPart = ~0u;  // part index of a split product's spectrum item
 constexpr int kMinPieceLog = 10;
 static int split_plan(u64 n1, u64 n2, int L, u64* m_out, u32* pieces) {
     if (L <= tuning(L).row_tile) return 0;  // one row pass already
@@ -1351,7 +1354,14 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
             const u64* la = d.na >= d.nb ? d.a : d.b;
             const u64* sb = d.na >= d.nb ? d.b : d.a;
             const u64 ln = std::max(d.na, d.nb), sn = std::min(d.na, d.nb);
-            j.specs.push_back(SpecStep{sb, sn, k, 0, nullptr});
+            ProductDesc f{};
+            f.kind = kKindSpecFwd;
+            f.a = f.b = sb;
+            f.na = f.nb = sn;
+            f.log_n = (u32)k;
+            items.push_back(f);
+            owner.push_back(ci);
+            part.push_back(kSpecPart);
             for (u32 q = 0; q < K; q++) {
                 ProductDesc c{};
                 c.kind = kKindMulSpec;
@@ -1377,11 +1387,13 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
         }
         j.crt.push_back(d);
     }
-    // transforms sorted by class; the pieces of one product stay adjacent
+    // transforms sorted by class, spectra of split products first; the
+    // pieces of one product stay adjacent
     std::vector<u32> order(items.size());
     for (u32 i = 0; i < (u32)order.size(); i++) order[i] = i;
+    auto key = [&](u32 x) { return items[x].kind == kKindSpecFwd ? -1 : (int)items[x].kind; };
     std::stable_sort(order.begin(), order.end(), [&](u32 x, u32 y) {
-        return items[x].kind != items[y].kind ? items[x].kind < items[y].kind : items[x].log_n < items[y].log_n;
+        return key(x) != key(y) ? key(x) < key(y) : items[x].log_n < items[y].log_n;
     });
     j.crt_res_off.assign(j.crt.size(), 0);
     j.items.resize(items.size());
@@ -1396,18 +1408,13 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
         j.res_words += (u64)(d.kind == kKindMul ? 6 : 3) << d.log_n;
         j.max_log_n = std::max(j.max_log_n, (int)d.log_n);
     }
-    // spectra of split products, after the residues
-    size_t sp = 0;
+    j.item_spec.assign(items.size(), 0);
+    for (size_t i = 0; i < order.size(); i++) j.item_spec[i] = part[order[i]] == kSpecPart;
     for (size_t c = 0; c < j.crt.size(); c++) {
         ProductDesc& d = j.crt[c];
         d.chunk0 = j.chunks;
         j.chunks += d.nchunks;
         for (u32 q = 0; q < d.nchunks; q++) j.chunk_item.push_back((u32)c);
-        if (d.split_n) {
-            SpecStep& st = j.specs[sp++];
-            st.off = j.res_words;
-            j.res_words += (u64)3 << st.log_n;
-        }
     }
     return 0;
 }
@@ -1432,18 +1439,16 @@ void job_bind(Job& j, void* ws) {
     char* p = (char*)ws;
     u64* res = (u64*)p;
     p += align256(j.res_words * sizeof(u64));
-    // residues; the pieces of a split product read its spectrum
+    // residues; the pieces of a split product read the spectrum its
+    // kKindSpecFwd item writes
     std::vector<const u64*> spec_of(j.crt.size(), nullptr);
-    size_t sp = 0;
-    for (auto& st : j.specs) st.words = res + st.off;
-    for (size_t c = 0; c < j.crt.size(); c++) {
-        j.crt[c].res = res + j.crt_res_off[c];
-        if (j.crt[c].split_n) spec_of[c] = j.specs[sp++].words;
-    }
+    for (size_t c = 0; c < j.crt.size(); c++) j.crt[c].res = res + j.crt_res_off[c];
     for (size_t i = 0; i < j.items.size(); i++) {
         j.items[i].res = res + j.res_off[i];
-        if (spec_of[j.item_owner[i]]) j.items[i].spec = spec_of[j.item_owner[i]];
+        if (j.item_spec[i]) spec_of[j.item_owner[i]] = j.items[i].res;
     }
+    for (size_t i = 0; i < j.items.size(); i++)
+        if (!j.item_spec[i] && spec_of[j.item_owner[i]]) j.items[i].spec = spec_of[j.item_owner[i]];
     j.fmap = (u8*)p;
     p += align256((size_t)j.chunks * kCarryChunk);
     j.chunk_map = (u32*)p;
@@ -1489,7 +1494,7 @@ hipError_t job_upload(const Job& j, hipStream_t s) {
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
-    const Tuning tu = tuning(log_n);
+    const Tuning tu = tuning(log_n, count >= kBatchedClass);
     const u32 kind = d0.kind;
     const u64 N = 1ull << log_n;
     const Plan pl = make_plan(log_n, tu);
@@ -1525,8 +1530,9 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         R.log_n = log_n;
         R.s0 = pl.row_s;
         R.from_limbs = (pl.n_col == 0);
-        static const char* const kRowName[4] = {"row_mul", "row_sqr", "row_mul_spec", "row_sqr_spec"};
-        static const int kRowModeOf[4] = {kRowMul, kRowSqr, kRowMulSpec, kRowSqrSpec};
+        static const char* const kRowName[5] = {"row_mul", "row_sqr", "row_mul_spec", "row_sqr_spec",
+                                                "row_fwd_keep"};
+        static const int kRowModeOf[5] = {kRowMul, kRowSqr, kRowMulSpec, kRowSqrSpec, kRowFwdKeep};
         R.mode = kRowModeOf[kind];
         const int logL = log_n - pl.row_s;
         const u64 lines = (u64)count << pl.row_s;
@@ -1542,7 +1548,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         else
             FMH_LAUNCH(kRowName[kind], k_row<4>, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
     }
-    for (int pi = pl.n_col - 1; pi >= 0; pi--) {
+    for (int pi = pl.n_col - 1; kind != kKindSpecFwd && pi >= 0; pi--) {
         ColArgs C{};
         C.items = items;
         C.one = d0;
@@ -1562,9 +1568,6 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     return hipSuccess;
 }
 
-static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int mode, const Twiddles& tw,
-                                  hipStream_t s);
-
 hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     prepare_kernels();
     if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
@@ -1574,11 +1577,6 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     }
     if (j.items.empty()) return hipSuccess;
     if (job_desc_bytes(j) && !j.d_desc) return hipErrorInvalidValue;
-    // spectra of the short operands of split products
-    for (const SpecStep& st : j.specs) {
-        hipError_t e = enqueue_forward(st.b, st.nb, st.log_n, st.words, kRowFwdKeep, tw, s);
-        if (e != hipSuccess) return e;
-    }
     constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
     size_t first = 0;
     while (first < j.items.size()) {

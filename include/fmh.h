@@ -177,19 +177,26 @@ int fmh_memcpy_d2h(void* dst, const void* src, u64 bytes, fmh_stream_t stream);
 int fmh_stream_create(fmh_stream_t* stream);
 int fmh_stream_destroy(fmh_stream_t stream);
 int fmh_stream_synchronize(fmh_stream_t stream);
+/* Waits for the current device, then frees the library's cached scratch
+ * (per-stream workspaces, staging of the host-pointer calls).  They are
+ * otherwise kept, grown as needed, until fmh_stream_destroy or process exit.
+ * Twiddle tables stay (planned batches may refer to them). */
+int fmh_trim(void);
 /* Plain word copy on the device, launched as kernel "copy": the known-size
  * reference kernel against which profiler byte counters are calibrated. */
 int fmh_copy_device(u64* d_dst, const u64* d_src, u64 n, fmh_stream_t stream);
 const char* fmh_last_error(void);
 
-/* Pass geometry of transforms of 2^log_n points (log_n = -1: every size):
- * spec "col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,col_threads=512,
- * row_threads=512" (missing keys keep their value; "" restores the
- * defaults).  Defaults come from tuning_gfx950.txt beside the library
- * (written by fast_mul_ntt_amd/autotune.py) or $FMH_TUNE_TABLE; $FMH_TUNE
- * overrides every size.  Results never depend on the geometry. */
-int fmh_set_tuning(int log_n, const char* spec);
-int fmh_get_tuning(int log_n, char* buf, u64 buflen);
+/* Pass geometry of transforms of 2^log_n points (log_n = -1: every size),
+ * for single products (batched = 0) or for launches carrying several
+ * products of the class (batched = 1): spec "col_logc=3,col_maxl=10,
+ * row_maxl=11,row_tile=12,col_threads=512,row_threads=512,row_radix=4"
+ * (missing keys keep their value; "" restores the defaults).  Defaults come
+ * from tuning_gfx950.txt beside the library (written by
+ * fast_mul_ntt_amd/autotune.py) or $FMH_TUNE_TABLE; $FMH_TUNE overrides
+ * every size.  Results never depend on the geometry. */
+int fmh_set_tuning(int log_n, const char* spec, int batched);
+int fmh_get_tuning(int log_n, int batched, char* buf, u64 buflen);
 
 /* Per-kernel timing with HIP events recorded on each kernel's launch stream
  * (for benchmarks).  fmh_timing_collect writes "name count total_ms" lines

@@ -60,14 +60,17 @@ def test_unbalanced_products_vs_oracle(fm, orc, n1, n2):
     assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
 
 
+def _tuning_specs():
+    from fast_mul_ntt_amd.autotune import CANDIDATES
+    specs = [f"{k}={v}" for k, vs in CANDIDATES.items() for v in vs]
+    return specs + ["row_maxl=8,row_tile=10,col_maxl=7,col_logc=4",
+                    "row_tile=13,row_maxl=12,col_logc=2,col_threads=1024,row_threads=1024"]
+
+
 def test_every_tuning_candidate_gives_the_same_product(fm, orc):
     """Each geometry the autotuner may choose (one parameter changed at a
     time, and two extreme combinations), on transforms of 2^14 and 2^17
     points: products equal the oracle's."""
-    from fast_mul_ntt_amd.autotune import CANDIDATES
-    specs = [f"{k}={v}" for k, vs in CANDIDATES.items() for v in vs]
-    specs += ["row_maxl=8,row_tile=10,col_maxl=7,col_logc=4",
-              "row_tile=13,row_maxl=12,col_logc=2,col_threads=1024,row_threads=1024"]
     cases = []
     for lg in (14, 17):
         n = 1 << (lg - 1)
@@ -75,7 +78,7 @@ def test_every_tuning_candidate_gives_the_same_product(fm, orc):
         cases.append((lg, a, b, orc.mul(a, b)))
     saved = {lg: fm.get_tuning(lg) for lg, *_ in cases}
     try:
-        for spec in specs:
+        for spec in _tuning_specs():
             for lg, a, b, want in cases:
                 fm.set_tuning(lg, "")
                 fm.set_tuning(lg, spec)
@@ -83,6 +86,31 @@ def test_every_tuning_candidate_gives_the_same_product(fm, orc):
     finally:
         for lg, t in saved.items():
             fm.set_tuning(lg, t)
+
+
+def test_every_batched_tuning_candidate_gives_the_same_products(fm, orc):
+    """The same geometries applied to batched launches: a batch of 5
+    products of the 2^14 class."""
+    bops = [(orc.splitmix(700 + i, 8192), orc.splitmix(800 + i, 8192 - i)) for i in range(5)]
+    bwant = [orc.mul(a, b) for a, b in bops]
+    bufs = [(fm.DeviceBuffer.from_host(a), fm.DeviceBuffer.from_host(b), fm.DeviceBuffer(a.size + b.size))
+            for a, b in bops]
+    items = [(da, a.size, db, b.size, do) for (a, b), (da, db, do) in zip(bops, bufs)]
+    saved = fm.get_tuning(14, batched=True)
+    try:
+        for spec in _tuning_specs():
+            fm.set_tuning(14, "", batched=True)
+            fm.set_tuning(14, spec, batched=True)
+            with fm.Stream() as s:
+                fm.mul_batch_device(items, s)
+                s.synchronize()
+            for (_, _, do), w in zip(bufs, bwant):
+                assert np.array_equal(do.download(), w), ("batched", spec)
+    finally:
+        fm.set_tuning(14, saved, batched=True)
+        for trio in bufs:
+            for d in trio:
+                d.free()
 
 
 @pytest.mark.parametrize("n", [4097, 65536, 1 << 20])
@@ -149,6 +177,13 @@ def test_forward_spectrum_vs_oracle(fm, orc, log_n):
         got = fm.forward_spectrum(prime, limbs, n)
         want = orc.forward_spectrum(prime, limbs, n)
         assert np.array_equal(got, want), (prime, log_n)
+
+
+def test_trim_then_products_still_exact(fm, orc):
+    a, b = orc.splitmix(61, 30000), orc.splitmix(62, 20000)
+    assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
+    assert fm.lib().fmh_trim() == 0
+    assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
 
 
 def test_device_entry_points(fm, orc):
