@@ -39,11 +39,11 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against the copy
 # kernel) from rocprofv3 --pmc runs of this workload; profiles/ holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 202454016.0,
-    "row_mul": 152238080.0,
-    "col_inv": 151102464.0,
-    "crt_local": 70256640.0,
-    "carry_apply": 34788352.0,
+    "col_fwd": 204603392.0,
+    "row_mul": 154431488.0,
+    "col_inv": 151078912.0,
+    "crt_local": 70309888.0,
+    "carry_apply": 34800640.0,
 }
 
 
@@ -230,11 +230,13 @@ def serve_requests(count: int = 20000, seed: int = 5001):
 
 def run_serve(args, fm) -> dict:
     """20,000 multiply/square requests through the request loop (64 GiB pool,
-    continuous batching), against plain per-request device calls."""
+    continuous batching), against plain per-request device calls and against
+    the reference's interface (host-pointer abs_mul64 per request, sampled)."""
     count = args.requests
     l1, l2, sq, o1, o2, gaps, src_len = serve_requests(count)
     stream = fm.Stream()
-    src = fm.DeviceBuffer.from_host(splitmix64(5001, src_len), stream)
+    src_host = splitmix64(5001, src_len)
+    src = fm.DeviceBuffer.from_host(src_host, stream)
     ops = []
     for i in range(count):
         a = src.view(int(o1[i]), int(l1[i]))
@@ -259,6 +261,18 @@ def run_serve(args, fm) -> dict:
     stream.synchronize()
     t_plain = time.perf_counter() - t0
     plain_rate = count / t_plain
+
+    # The reference's own interface: host-pointer abs_mul64 / abs_sqr64 per
+    # request (PCIe both ways included), on every 40th request.
+    host_idx = list(range(0, count, 40))
+    t0 = time.perf_counter()
+    for i in host_idx:
+        a = src_host[o1[i]:o1[i] + l1[i]]
+        if sq[i]:
+            fm.abs_sqr64(a)
+        else:
+            fm.abs_mul64(a, src_host[o2[i]:o2[i] + l2[i]])
+    host_rate = len(host_idx) / (time.perf_counter() - t0)
 
     srv = fm.Server(pool_bytes=args.pool_gib << 30, streams=4)
     check_every = max(1, count // 40)
@@ -323,6 +337,10 @@ def run_serve(args, fm) -> dict:
         "plain_products_per_s": round(plain_rate, 1),
         "speedup_vs_plain": round(st["products_per_s"] / plain_rate, 2),
         "capacity_vs_plain": round(cap["products_per_s"] / plain_rate, 2),
+        "plain_host_abi_products_per_s": round(host_rate, 1),
+        "plain_host_abi_sample": f"every 40th request ({len(host_idx)}), host-pointer abs_mul64/abs_sqr64",
+        "speedup_vs_plain_host_abi": round(st["products_per_s"] / host_rate, 2),
+        "capacity_vs_plain_host_abi": round(cap["products_per_s"] / host_rate, 2),
         "mean_group": round(st["mean_group"], 2),
         "groups": st["groups"],
         "pool_gib": args.pool_gib,
