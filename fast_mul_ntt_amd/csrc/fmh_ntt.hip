@@ -1064,10 +1064,13 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
 // maps pass through LDS in tiles of kScanTile (coalesced loads); within a
 // tile, thread t owns a contiguous segment, and the carry out of each tile
 // enters the next.
-constexpr int kScanTile = 4096;
+#ifndef FMH_SCAN_TILE
+#define FMH_SCAN_TILE 4096  // the CPU emulation builds with a small tile to reach the multi-tile path
+#endif
+constexpr int kScanTile = FMH_SCAN_TILE;
 
-__global__ __launch_bounds__(kThreads) void k_chunk_scan(CarryArgs A) {
-    __shared__ u32 s_wave[kThreads / 64];
+__global__ __launch_bounds__(kScanThreads) void k_chunk_scan(CarryArgs A) {
+    __shared__ u32 s_wave[kScanThreads / 64];
     __shared__ u32 s_m[kScanTile];
     const int tid = threadIdx.x;
     const ProductDesc& D = product(A.items, A.one, blockIdx.x);
@@ -1596,7 +1599,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     K.chunk_map = j.chunk_map;
     K.chunk_cin = j.chunk_cin;
     FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kThreads), 0, s, K);
-    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.crt.size()), dim3(kThreads), 0, s, K);
+    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.crt.size()), dim3(kScanThreads), 0, s, K);
     FMH_LAUNCH("carry_apply", k_carry_apply, dim3(j.chunks), dim3(kThreads), 0, s, K);
     return hipSuccess;
 }

@@ -328,6 +328,26 @@ def test_resident_spectrum_shared_by_two_streams(fm, orc):
         s2.close()
 
 
+def test_largest_transform_and_refusal_beyond(fm, orc):
+    """A 2^27-limb square fills the largest transform (2^28 points): checked
+    through residues mod random 61-bit primes; one coefficient more is
+    refused before anything runs."""
+    import ctypes
+    n = 1 << 27
+    a = orc.splitmix(2718, n)
+    got = fm.abs_sqr64(a)
+    assert got.size == 2 * n
+    assert int(got[0]) == (int(a[0]) * int(a[0])) & 0xFFFFFFFFFFFFFFFF
+    for q in _primes61(2, 77):
+        qa = orc.mod_q(a, q)
+        assert orc.mod_q(got, q) == orc.mulmod_q(qa, qa, q), q
+    assert fm.lib().fmh_max_ntt_len() == 1 << 28
+    with fm.DeviceBuffer(16) as d, fm.Stream() as s:
+        # lengths only: the library refuses them before touching memory
+        rc = fm.lib().fmh_mul_device(d.ptr, n + 1, d.ptr + 8, n + 1, d.ptr, s.handle)
+        assert rc != 0 and b"fmh_max_ntt_len" in fm.lib().fmh_last_error()
+
+
 def _primes61(count, seed):
     import random
     rng = random.Random(seed)
