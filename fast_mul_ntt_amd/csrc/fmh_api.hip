@@ -11,8 +11,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -28,12 +30,93 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// Memory copies spread over a few persistent host threads (the calling
+// thread takes one part), for moving large pageable buffers into and out of
+// pinned staging memory.
+class CopyPool {
+   public:
+    CopyPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int workers = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+        for (int i = 0; i < workers; i++) th_.emplace_back([this, i] { work(i + 1); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void copy(void* dst, const void* src, size_t n) {
+        if (th_.empty() || n < (1u << 20)) {
+            memcpy(dst, src, n);
+            return;
+        }
+        std::unique_lock<std::mutex> lk(mu_);
+        dst_ = (char*)dst;
+        src_ = (const char*)src;
+        n_ = n;
+        parts_ = (int)th_.size() + 1;
+        pending_ = (int)th_.size();
+        gen_++;
+        lk.unlock();
+        cv_.notify_all();
+        part(0);
+        lk.lock();
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+   private:
+    void part(int id) {
+        const size_t per = (n_ / parts_ + 63) & ~(size_t)63;
+        const size_t b = std::min(n_, per * (size_t)id), e = std::min(n_, b + per);
+        if (e > b) memcpy(dst_ + b, src_ + b, e - b);
+    }
+    void work(int id) {
+        unsigned long long seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            lk.unlock();
+            part(id);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t n_ = 0;
+    int parts_ = 1, pending_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+};
+
+CopyPool& copy_pool() {
+    static CopyPool pool;
+    return pool;
+}
+
+// Two pinned staging buffers of the host-pointer calls; event i marks the
+// last transfer that used buffer i.
+#ifndef FMH_PIN_CHUNK
+#define FMH_PIN_CHUNK (8u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
+#endif
+constexpr size_t kPinChunk = FMH_PIN_CHUNK;
+
 struct DevCtx {
     bool ready = false;
     hipStream_t stream = nullptr;  // private stream of the host-pointer API
     Twiddles tw;
     std::map<hipStream_t, Buffer> ws;
     Buffer stage_a, stage_b, stage_out;
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
 constexpr int kMaxDevices = 64;
@@ -160,6 +243,64 @@ int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool squa
     return run_job(c, &in, 1, s);
 }
 
+int ensure_pinned(DevCtx& c) {
+    for (int i = 0; i < 2; i++) {
+        if (c.pin[i]) continue;
+        hipError_t e = hipHostMalloc(&c.pin[i], kPinChunk, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c.pin_ev[i], hipEventDisableTiming);
+        if (e != hipSuccess) return set_err("pinned staging", e);
+    }
+    return 0;
+}
+
+// Pageable host -> device through the pinned buffers: while chunk k moves
+// over DMA, the host threads copy chunk k + 1 into the other buffer.
+int h2d_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
+    if (bytes < 2 * kPinChunk) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream);
+        return e == hipSuccess ? 0 : set_err("hipMemcpyAsync H2D", e);
+    }
+    if (ensure_pinned(c)) return -1;
+    for (size_t off = 0, i = 0; off < bytes; off += kPinChunk, i ^= 1) {
+        const size_t n = std::min(kPinChunk, bytes - off);
+        hipError_t e = hipEventSynchronize(c.pin_ev[i]);
+        if (e != hipSuccess) return set_err("hipEventSynchronize", e);
+        copy_pool().copy(c.pin[i], (const char*)src + off, n);
+        e = hipMemcpyAsync((char*)dst + off, c.pin[i], n, hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess) e = hipEventRecord(c.pin_ev[i], c.stream);
+        if (e != hipSuccess) return set_err("hipMemcpyAsync H2D", e);
+    }
+    return 0;
+}
+
+// Device -> pageable host: chunk k + 1 is in flight over DMA while the host
+// threads copy chunk k out of its buffer.  Returns after the last byte.
+int d2h_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
+    hipError_t e;
+    if (bytes < 2 * kPinChunk) {
+        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        return e == hipSuccess ? 0 : set_err("hipMemcpyAsync D2H", e);
+    }
+    if (ensure_pinned(c)) return -1;
+    const size_t chunks = (bytes + kPinChunk - 1) / kPinChunk;
+    auto issue = [&](size_t k) {
+        const size_t off = k * kPinChunk, n = std::min(kPinChunk, bytes - off);
+        hipError_t r = hipMemcpyAsync(c.pin[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, c.stream);
+        if (r == hipSuccess) r = hipEventRecord(c.pin_ev[k & 1], c.stream);
+        return r;
+    };
+    e = issue(0);
+    for (size_t k = 0; e == hipSuccess && k < chunks; k++) {
+        if (k + 1 < chunks) e = issue(k + 1);  // its buffer was emptied by the copy of chunk k - 1
+        if (e == hipSuccess) e = hipEventSynchronize(c.pin_ev[k & 1]);
+        if (e != hipSuccess) break;
+        const size_t off = k * kPinChunk;
+        copy_pool().copy((char*)dst + off, c.pin[k & 1], std::min(kPinChunk, bytes - off));
+    }
+    return e == hipSuccess ? 0 : set_err("staged D2H", e);
+}
+
 [[noreturn]] void die(const char* fn) {
     fprintf(stderr, "%s: %s\n", fn, t_err.c_str());
     abort();
@@ -181,29 +322,10 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     u64* da = (u64*)c->stage_a.ptr;
     u64* db = square ? da : (u64*)c->stage_b.ptr;
     u64* dout = (u64*)c->stage_out.ptr;
-    hipError_t e = hipMemcpyAsync(da, in1, len1 * sizeof(u64), hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) {
-        set_err("hipMemcpyAsync H2D", e);
-        die(fn);
-    }
-    if (!square) {
-        e = hipMemcpyAsync(db, in2, len2 * sizeof(u64), hipMemcpyHostToDevice, c->stream);
-        if (e != hipSuccess) {
-            set_err("hipMemcpyAsync H2D", e);
-            die(fn);
-        }
-    }
+    if (h2d_staged(*c, da, in1, len1 * sizeof(u64))) die(fn);
+    if (!square && h2d_staged(*c, db, in2, len2 * sizeof(u64))) die(fn);
     if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, c->stream)) die(fn);
-    e = hipMemcpyAsync(out, dout, out_len * sizeof(u64), hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) {
-        set_err("hipMemcpyAsync D2H", e);
-        die(fn);
-    }
-    e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        set_err("hipStreamSynchronize", e);
-        die(fn);
-    }
+    if (d2h_staged(*c, out, dout, out_len * sizeof(u64))) die(fn);
 }
 
 }  // namespace
