@@ -105,7 +105,7 @@ CopyPool& copy_pool() {
 // Two pinned staging buffers of the host-pointer calls; event i marks the
 // last transfer that used buffer i.
 #ifndef FMH_PIN_CHUNK
-#define FMH_PIN_CHUNK (8u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
+#define FMH_PIN_CHUNK (4u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
 #endif
 constexpr size_t kPinChunk = FMH_PIN_CHUNK;
 
@@ -253,13 +253,11 @@ int ensure_pinned(DevCtx& c) {
     return 0;
 }
 
-// Pageable host -> device through the pinned buffers: while chunk k moves
-// over DMA, the host threads copy chunk k + 1 into the other buffer.
+// Pageable host -> device through the pinned buffers (a pageable
+// hipMemcpy costs hundreds of microseconds even for small buffers): while
+// chunk k moves over DMA, the host threads copy chunk k + 1 into the other
+// buffer.
 int h2d_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
-    if (bytes < 2 * kPinChunk) {
-        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream);
-        return e == hipSuccess ? 0 : set_err("hipMemcpyAsync H2D", e);
-    }
     if (ensure_pinned(c)) return -1;
     for (size_t off = 0, i = 0; off < bytes; off += kPinChunk, i ^= 1) {
         const size_t n = std::min(kPinChunk, bytes - off);
@@ -277,11 +275,6 @@ int h2d_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
 // threads copy chunk k out of its buffer.  Returns after the last byte.
 int d2h_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
     hipError_t e;
-    if (bytes < 2 * kPinChunk) {
-        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-        return e == hipSuccess ? 0 : set_err("hipMemcpyAsync D2H", e);
-    }
     if (ensure_pinned(c)) return -1;
     const size_t chunks = (bytes + kPinChunk - 1) / kPinChunk;
     auto issue = [&](size_t k) {
