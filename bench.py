@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "µs per 2^20×2^20-limb abs_mul64 (bit-exact); mixed-size products/s + p99"
+METRIC_16M = "µs per 2^24×2^24-limb abs_mul64 (bit-exact)"
 N1 = N2 = 1 << 20
 SEED_A, SEED_B = 2001, 2002
 HBM_PEAK_GBS = 8000.0
@@ -44,6 +45,15 @@ TRAFFIC: dict[str, float] = {
     "col_inv": 151078912.0,
     "crt_local": 70309888.0,
     "carry_apply": 34800640.0,
+}
+# The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
+# the two column passes).
+TRAFFIC_16M: dict[str, float] = {
+    "col_fwd": 3592051712.0,
+    "row_mul": 2418100224.0,
+    "col_inv": 2039309312.0,
+    "crt_local": 1121181696.0,
+    "carry_apply": 555824128.0,
 }
 
 
@@ -61,13 +71,16 @@ def splitmix64(seed: int, n: int, top_nonzero: bool = True) -> np.ndarray:
     return z
 
 
-def kernel_alg_bytes(n1: int, n2: int, log_n: int) -> dict[str, float]:
-    """Algorithmic HBM bytes per launch of each kernel of one multiply.
+def kernel_alg_bytes(n1: int, n2: int, log_n: int, col_passes: int = 1) -> dict[str, float]:
+    """Algorithmic HBM bytes per launch of each kernel of one multiply
+    (averaged over the launches of a kernel when there are several column
+    passes).
 
-    col_fwd     reads both operands' limbs, writes 3 primes x 2 operands x N words
+    col_fwd     the first pass reads both operands' limbs and writes 3 primes
+                x 2 operands x N words; each further pass reads and writes them
     row_mul     reads those 6 residue arrays, writes 3 (the pointwise product,
                 inverse row stages done)
-    col_inv     reads and writes 3 residue arrays
+    col_inv     each pass reads and writes 3 residue arrays
     crt_local   reads 3 residues per convolution coefficient, writes the
                 limbs and one carry-map byte per limb
     carry_apply reads the carry-map bytes and adds the carries into the limbs
@@ -76,8 +89,9 @@ def kernel_alg_bytes(n1: int, n2: int, log_n: int) -> dict[str, float]:
     N = 1 << log_n
     conv = n1 + n2 - 1
     out = n1 + n2
+    P = max(1, col_passes)
     return {
-        "col_fwd": 8.0 * (n1 + n2) + 6 * 8.0 * N,
+        "col_fwd": (8.0 * (n1 + n2) + 6 * 8.0 * N + (P - 1) * 12 * 8.0 * N) / P,
         "row_mul": 9 * 8.0 * N,
         "col_inv": 6 * 8.0 * N,
         "crt_local": 3 * 8.0 * conv + 9.0 * out,
@@ -406,6 +420,155 @@ def run_sizes(args, fm) -> dict:
             "data": "synthetic (splitmix64, seeds 1000*cfg+k)", "config": {"workload": "single products"}}
 
 
+def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str,
+               traffic: dict[str, float]) -> dict:
+    """One device-resident n1 x n2 multiply per step (the headline shape by
+    default); per-kernel breakdown, roofline of the dominant kernel, host-ABI
+    time and the CPU baselines on the same operands."""
+    a = splitmix64(seed_a, n1)
+    b = splitmix64(seed_b, n2)
+    log_n = (n1 + n2 - 2).bit_length()
+    stream = fm.Stream()
+    da = fm.DeviceBuffer.from_host(a, stream)
+    db = fm.DeviceBuffer.from_host(b, stream)
+    dout = fm.DeviceBuffer(n1 + n2)
+
+    for _ in range(args.warmup):
+        fm.mul_device(da, n1, db, n2, dout, stream)
+    stream.synchronize()
+    fm.timing_collect()
+
+    # Breakdown pass (not the timed region): events around every kernel.
+    n_break = max(3, args.steps // 4)
+    fm.timing_enable(True)
+    for _ in range(n_break):
+        fm.mul_device(da, n1, db, n2, dout, stream)
+    stream.synchronize()
+    fm.timing_enable(False)
+    breakdown = fm.timing_collect()
+    passes = round(breakdown.get("col_fwd", (n_break, 0.0))[0] / n_break)
+    alg = kernel_alg_bytes(n1, n2, log_n, passes)
+    dom = max((k for k in breakdown if k in alg),
+              key=lambda k: breakdown[k][1] / breakdown[k][0])
+
+    # Timed region: K steps; events only around the dominant kernel, so that
+    # they add no gaps between the other launches.
+    fm.timing_enable(True, only=dom)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fm.mul_device(da, n1, db, n2, dout, stream)
+    stream.synchronize()
+    t1 = time.perf_counter()
+    fm.timing_enable(False)
+    kt = fm.timing_collect()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    got = dout.download(n1 + n2)
+    if args.copy_calib:
+        words = 1 << 25
+        with fm.DeviceBuffer(words) as cs, fm.DeviceBuffer(words) as cd:
+            fm.copy_device(cd, cs, words, stream)
+            stream.synchronize()
+
+    # Host-pointer drop-in (PCIe included), reported beside the device number.
+    t_host = []
+    for _ in range(3):
+        h0 = time.perf_counter()
+        host_out = fm.abs_mul64(a, b)
+        t_host.append(time.perf_counter() - h0)
+    if not np.array_equal(host_out, got):
+        raise AssertionError("host-pointer abs_mul64 differs from the device-pointer result")
+
+    per_kernel = {}
+    for name, (cnt, tot_ms) in breakdown.items():
+        avg_us = tot_ms * 1e3 / cnt
+        entry = {"launches_per_step": cnt / n_break, "avg_us": round(avg_us, 3)}
+        if name in alg:
+            entry["alg_bytes"] = alg[name]
+            entry["GBps"] = round(alg[name] / (avg_us * 1e-6) / 1e9, 1)
+        if name in traffic:
+            entry["traffic_bytes"] = traffic[name]
+        per_kernel[name] = entry
+    kernel_sum_us = sum(v["avg_us"] * v["launches_per_step"] for v in per_kernel.values())
+    dom_cnt, dom_ms = kt[dom]
+    dom_us = dom_ms * 1e3 / dom_cnt
+    dom_gbs = alg[dom] / (dom_us * 1e-6) / 1e9
+    mm = modmul_count(n1, n2, log_n)
+
+    result = {
+        "metric": metric,
+        "value": round(ms_per_step * 1e3, 2),
+        "unit": "µs",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": f"synthetic (splitmix64 seeds {seed_a}/{seed_b}, top limb nonzero)",
+        "config": {
+            "workload": f"abs_mul64 {n1}x{n2} limbs, device-resident",
+            "ntt_len": 1 << log_n,
+            "column_passes": passes,
+            "primes": 3,
+            "products_per_step": 1,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "kernel_avg_us": round(dom_us, 3),
+            "launches_timed": dom_cnt,
+            "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get(dom),
+        },
+        "valu": {
+            "modmuls_per_product": mm,
+            "Gmodmul_per_s": round(mm / (ms_per_step * 1e-3) / 1e9, 2),
+            "lane_ops_per_modmul_at_peak": round(VALU_LANE_OPS * ms_per_step * 1e-3 / mm, 1),
+        },
+        "kernels": per_kernel,
+        "kernel_sum_us": round(kernel_sum_us, 2),
+        "host_abi_us": round(min(t_host) * 1e6, 1),
+    }
+
+    if not args.no_cpu_baseline:
+        want, times = cpu_baseline(a, b)
+        if not np.array_equal(got, want):
+            raise AssertionError("GPU product differs from the CPU oracle")
+        result["bit_exact_vs_oracle"] = True
+        result["cpu_baseline"] = {
+            "value": round(float(np.mean(times)) * 1e6, 1),
+            "median": round(float(np.median(times)) * 1e6, 1),
+            "unit": "µs",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{len(times)} x the same {n1}x{n2}-limb abs_mul64 (oracle, 1 thread)",
+        }
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+            result["cpu_baseline"]["cpu"] = model
+        except Exception:
+            pass
+        fs = cpu_fft_small(a, b, want)
+        if fs is not None:
+            result["cpu_fft_small"] = {
+                "value": round(float(np.mean(fs)) * 1e6, 1),
+                "median": round(float(np.median(fs)) * 1e6, 1),
+                "unit": "µs",
+                "cores": 1,
+                "kind": "reference",
+                "sample": f"{len(fs)} x the same {n1}x{n2}-limb abs_mul64 of the reference's "
+                          "fft_small (built from its sources, 1 thread, after one warm-up call)",
+            }
+    return result
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,10 +577,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")
-    ap.add_argument("--workload", default="mul1m", choices=["mul1m", "batch", "sizes", "serve"],
-                    help="mul1m: the 2^20 x 2^20 headline (default); batch: 4,096 mixed products; "
-                         "sizes: every single-product configuration; serve: 20,000 requests "
-                         "through the request loop")
+    ap.add_argument("--workload", default="mul1m", choices=["mul1m", "mul16m", "batch", "sizes", "serve"],
+                    help="mul1m: the 2^20 x 2^20 headline (default); mul16m: the 2^24 x 2^24 product "
+                         "measured the same way; batch: 4,096 mixed products; sizes: every "
+                         "single-product configuration; serve: 20,000 requests through the request loop")
     ap.add_argument("--requests", type=int, default=20000, help="serve: number of requests")
     ap.add_argument("--load", type=float, default=0.7, help="serve: offered rate / capacity")
     ap.add_argument("--pool-gib", type=int, default=64, help="serve: device pool (GiB)")
@@ -441,146 +604,11 @@ def main() -> int:
     if args.workload == "serve":
         print(json.dumps(run_serve(args, fm)))
         return 0
-    a = splitmix64(SEED_A, N1)
-    b = splitmix64(SEED_B, N2)
-    log_n = (N1 + N2 - 2).bit_length()
-    stream = fm.Stream()
-    da = fm.DeviceBuffer.from_host(a, stream)
-    db = fm.DeviceBuffer.from_host(b, stream)
-    dout = fm.DeviceBuffer(N1 + N2)
-
-    for _ in range(args.warmup):
-        fm.mul_device(da, N1, db, N2, dout, stream)
-    stream.synchronize()
-    fm.timing_collect()
-
-    # Breakdown pass (not the timed region): events around every kernel.
-    alg = kernel_alg_bytes(N1, N2, log_n)
-    fm.timing_enable(True)
-    for _ in range(max(3, args.steps // 4)):
-        fm.mul_device(da, N1, db, N2, dout, stream)
-    stream.synchronize()
-    fm.timing_enable(False)
-    breakdown = fm.timing_collect()
-    dom = max((k for k in breakdown if k in alg),
-              key=lambda k: breakdown[k][1] / breakdown[k][0])
-
-    # Timed region: K steps; events only around the dominant kernel, so that
-    # they add no gaps between the other launches.
-    fm.timing_enable(True, only=dom)
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        fm.mul_device(da, N1, db, N2, dout, stream)
-    stream.synchronize()
-    t1 = time.perf_counter()
-    fm.timing_enable(False)
-    kt = fm.timing_collect()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    got = dout.download(N1 + N2)
-    if args.copy_calib:
-        words = 1 << 25
-        with fm.DeviceBuffer(words) as cs, fm.DeviceBuffer(words) as cd:
-            fm.copy_device(cd, cs, words, stream)
-            stream.synchronize()
-
-    # Host-pointer drop-in (PCIe included), reported beside the device number.
-    t_host = []
-    for _ in range(3):
-        h0 = time.perf_counter()
-        host_out = fm.abs_mul64(a, b)
-        t_host.append(time.perf_counter() - h0)
-    if not np.array_equal(host_out, got):
-        raise AssertionError("host-pointer abs_mul64 differs from the device-pointer result")
-
-    per_kernel = {}
-    n_break = max(3, args.steps // 4)
-    for name, (cnt, tot_ms) in breakdown.items():
-        avg_us = tot_ms * 1e3 / cnt
-        entry = {"launches_per_step": cnt / n_break, "avg_us": round(avg_us, 3)}
-        if name in alg:
-            entry["alg_bytes"] = alg[name]
-            entry["GBps"] = round(alg[name] / (avg_us * 1e-6) / 1e9, 1)
-        if name in TRAFFIC:
-            entry["traffic_bytes"] = TRAFFIC[name]
-        per_kernel[name] = entry
-    kernel_sum_us = sum(v["avg_us"] * v["launches_per_step"] for v in per_kernel.values())
-    dom_cnt, dom_ms = kt[dom]
-    dom_us = dom_ms * 1e3 / dom_cnt
-    dom_gbs = alg[dom] / (dom_us * 1e-6) / 1e9
-    mm = modmul_count(N1, N2, log_n)
-
-    result = {
-        "metric": METRIC,
-        "value": round(ms_per_step * 1e3, 2),
-        "unit": "µs",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic (splitmix64 seeds 2001/2002, top limb nonzero)",
-        "config": {
-            "workload": "abs_mul64 1048576x1048576 limbs, device-resident",
-            "ntt_len": 1 << log_n,
-            "primes": 3,
-            "products_per_step": 1,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "kernel_avg_us": round(dom_us, 3),
-            "launches_timed": dom_cnt,
-            "achieved": round(dom_gbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-            "traffic": TRAFFIC.get(dom),
-        },
-        "valu": {
-            "modmuls_per_product": mm,
-            "Gmodmul_per_s": round(mm / (ms_per_step * 1e-3) / 1e9, 2),
-            "lane_ops_per_modmul_at_peak": round(VALU_LANE_OPS * ms_per_step * 1e-3 / mm, 1),
-        },
-        "kernels": per_kernel,
-        "kernel_sum_us": round(kernel_sum_us, 2),
-        "host_abi_us": round(min(t_host) * 1e6, 1),
-    }
-
-    if not args.no_cpu_baseline:
-        want, times = cpu_baseline(a, b)
-        if not np.array_equal(got, want):
-            raise AssertionError("GPU product differs from the CPU oracle")
-        result["bit_exact_vs_oracle"] = True
-        result["cpu_baseline"] = {
-            "value": round(float(np.mean(times)) * 1e6, 1),
-            "median": round(float(np.median(times)) * 1e6, 1),
-            "unit": "µs",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{len(times)} x the same 1048576x1048576-limb abs_mul64 (oracle, 1 thread)",
-        }
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
-            result["cpu_baseline"]["cpu"] = model
-        except Exception:
-            pass
-        fs = cpu_fft_small(a, b, want)
-        if fs is not None:
-            result["cpu_fft_small"] = {
-                "value": round(float(np.mean(fs)) * 1e6, 1),
-                "median": round(float(np.median(fs)) * 1e6, 1),
-                "unit": "µs",
-                "cores": 1,
-                "kind": "reference",
-                "sample": f"{len(fs)} x the same 1048576x1048576-limb abs_mul64 of the reference's "
-                          "fft_small (built from its sources, 1 thread, after one warm-up call)",
-            }
-    print(json.dumps(result))
+    if args.workload == "mul16m":
+        res = run_single(args, fm, 1 << 24, 1 << 24, 3001, 3002, METRIC_16M, TRAFFIC_16M)
+    else:
+        res = run_single(args, fm, N1, N2, SEED_A, SEED_B, METRIC, TRAFFIC)
+    print(json.dumps(res))
     return 0
 
 
