@@ -40,3 +40,17 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_reference_style_c_caller_links(tmp_path):
+    """A C program written against the reference's interface (tests/c_caller.c,
+    fmh.h in place of fast_mul.h) compiles and links against libfmh.so."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc needed")
+    exe = tmp_path / "c_caller"
+    subprocess.run(["gcc", "-O2", "-std=c11", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c_caller.c"), "-L", os.path.join(ROOT, "fast_mul_ntt_amd"),
+                    "-lfmh", "-o", str(exe)], check=True)
+    assert exe.exists()

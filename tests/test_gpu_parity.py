@@ -179,6 +179,18 @@ def test_forward_spectrum_vs_oracle(fm, orc, log_n):
         assert np.array_equal(got, want), (prime, log_n)
 
 
+def test_reference_style_c_caller_runs():
+    """tests/c_caller (built by __graft_entry__.build()): a C program using
+    only the reference's interface, linked against libfmh.so."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c_caller")
+    if not os.path.exists(exe):
+        pytest.fail("tests/c_caller missing: run __graft_entry__.build()")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "c caller ok" in r.stdout, r.stdout + r.stderr
+
+
 def test_trim_then_products_still_exact(fm, orc):
     a, b = orc.splitmix(61, 30000), orc.splitmix(62, 20000)
     assert np.array_equal(fm.abs_mul64(a, b), orc.mul(a, b))
