@@ -309,6 +309,42 @@ std::string timing_collect() {
         if (e_ != hipSuccess) return e_;                                         \
     } while (0)
 
+// ------------------------------------------------------- global memory access
+//
+// Operand and residue pointers reach the kernels through descriptors, so the
+// compiler cannot tell that they point to global memory and would emit flat
+// loads and stores; a wave's wait for its LDS accesses then also waits for
+// every flat access in flight.  gmem() states the address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FMH_GLOBAL __attribute__((address_space(1)))
+#else
+#define FMH_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ FMH_GLOBAL T* gmem(T* p) {
+    return (FMH_GLOBAL T*)p;
+}
+
+// Copies into / out of a tile move kBatch words per thread per step: every
+// load of a step is issued before the first one is used, so a wave waits for
+// one memory latency per step instead of one per word.  The loads of a step
+// are unconditional (an index past the end is clamped to the last word and
+// its value dropped), which keeps them in one basic block.
+constexpr int kBatch = 8;
+
+// For e in [0, n): put(e, get(min(e', n - 1))) ... with get() a plain load.
+template <class Get, class Put>
+__device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) {
+    for (int e0 = threadIdx.x; e0 < n; e0 += kBatch * nthr) {
+        u64 v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) v[k] = get(min(e0 + k * nthr, n - 1));
+#pragma unroll
+        for (int k = 0; k < kBatch; k++)
+            if (e0 + k * nthr < n) put(e0 + k * nthr, v[k]);
+    }
+}
+
 // ---------------------------------------------------------------- LDS tiles
 //
 // A tile holds C = 2^logC lines of L = 2^logL words.
@@ -321,13 +357,13 @@ std::string timing_collect() {
 template <int P, int MAP, int NT>
 __device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__ t1, int logL,
                                          int logC, int logB, u32 b0,
-                                         const u64* __restrict__ W) {
+                                         const u64* __restrict__ W, int nthr) {
     int rem = logL;
     while (rem >= 2) {
         const int lq = rem - 2;
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
-        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             int c, u, blk;
             if (MAP == 0) {
                 c = beta & ((1 << logC) - 1);
@@ -372,7 +408,7 @@ __device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__
     }
     if (rem == 1) {
         const int nbf = 1 << (logL - 1 + logC);
-        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             int c, blk;
             if (MAP == 0) {
                 c = beta & ((1 << logC) - 1);
@@ -406,10 +442,10 @@ __device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__
 
 template <int P, int MAP>
 __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
-                                         const u64* __restrict__ W) {
+                                         const u64* __restrict__ W, int nthr) {
     if (logL & 1) {
         const int nbf = 1 << (logL - 1 + logC);
-        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             int c, blk;
             if (MAP == 0) {
                 c = beta & ((1 << logC) - 1);
@@ -438,7 +474,7 @@ __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC
     for (int lq = logL & 1; lq + 2 <= logL; lq += 2) {
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
-        for (int beta = threadIdx.x; beta < nbf; beta += blockDim.x) {
+        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             int c, u, blk;
             if (MAP == 0) {
                 c = beta & ((1 << logC) - 1);
@@ -495,7 +531,8 @@ struct ColArgs {
     int log_n, s0, s1;
     int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
-    int nops;  // operands transformed per product: 2 (multiply) or 1
+    int nops;     // operands transformed per product: 2 (multiply) or 1
+    int threads;  // workgroup size
 };
 
 // Workgroups are dispatched to the 8 XCDs of the chip round-robin by
@@ -529,29 +566,31 @@ __device__ void col_body(const ColArgs& A) {
     const u64 a = lam0 >> logB;
     const u32 b0 = (u32)(lam0 & ((1ull << logB) - 1));
     const u64 base = (a << (logL + logB)) + b0;
-    u64* __restrict__ res = D.res + ((u64)(op * 3 + P) << A.log_n);
-    const u64* __restrict__ limbs = op ? D.b : D.a;
+    const int nthr = A.threads;
+    FMH_GLOBAL u64* __restrict__ res = gmem(D.res + ((u64)(op * 3 + P) << A.log_n));
+    const FMH_GLOBAL u64* __restrict__ limbs = gmem(op ? D.b : D.a);
     const u64 len = op ? D.nb : D.na;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int j = e >> logC, c = e & (C - 1);
-        const u64 g = base + ((u64)j << logB) + (u64)c;
-        u64 v;
-        if (A.from_limbs)
-            v = g < len ? reduce_word<P>(limbs[g]) : 0ull;
-        else
-            v = res[g];
-        lds[e] = v;
+    // tile word e = j C + c is word base + j B + c of the transform
+    auto at = [&](int e) { return base + ((u64)(e >> logC) << logB) + (u64)(e & (C - 1)); };
+    if (!A.from_limbs) {
+        batched_copy(n, nthr, [&](int e) { return res[at(e)]; }, [&](int e, u64 v) { lds[e] = v; });
+    } else if (len > 0) {
+        batched_copy(
+            n, nthr,
+            [&](int e) {
+                const u64 g = at(e);
+                return limbs[g < len ? g : len - 1];
+            },
+            [&](int e, u64 v) { lds[e] = at(e) < len ? reduce_word<P>(v) : 0ull; });
+    } else {
+        for (int e = threadIdx.x; e < n; e += nthr) lds[e] = 0;
     }
     __syncthreads();
     if (FWD)
-        tile_dif<P, 0, 1>(lds, nullptr, logL, logC, logB, b0, A.tw[P]);
+        tile_dif<P, 0, 1>(lds, nullptr, logL, logC, logB, b0, A.tw[P], nthr);
     else
-        tile_dit<P, 0>(lds, logL, logC, logB, b0, A.tw[P]);
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int j = e >> logC, c = e & (C - 1);
-        const u64 g = base + ((u64)j << logB) + (u64)c;
-        res[g] = lds[e];
-    }
+        tile_dit<P, 0>(lds, logL, logC, logB, b0, A.tw[P], nthr);
+    batched_copy(n, nthr, [&](int e) { return lds[e]; }, [&](int e, u64 v) { res[at(e)] = v; });
 }
 
 template <bool FWD>
@@ -591,16 +630,20 @@ struct RowArgs {
     int logC;  // 2^logC lines per tile
     int from_limbs;
     int mode;
+    int threads;  // workgroup size
 };
 
+// Where a line's operand words come from: src[op][min(g0 + j, last[op])],
+// reduced and zero from len[op] on when the pass starts from limbs.  A line
+// with nothing to read (inactive, empty operand) points at the first twiddle
+// with last = 0, so every load of the pass goes to a valid address.
 struct RowLine {
-    const u64* a;
-    const u64* b;
-    u64 na, nb;
-    u64* ra;
-    const u64* rb;
-    const u64* rs;  // resident spectrum of this prime (spec modes)
-    u64 g0;         // offset of the line inside its product's arrays
+    const u64* src[2];  // limbs, residues of the column passes, or the spectrum
+    u64 last[2];
+    u64 len[2];
+    u64* ra;            // residues of this prime, written back
+    const u64* rs;      // resident spectrum of this prime (spec modes)
+    u64 g0;             // offset of the line inside its product's arrays
     u32 active;
 };
 
@@ -659,9 +702,9 @@ __device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, i
 
 // A round over the tiles of nt operands (t1 = second operand or null).
 template <int P, int R, bool DIF>
-__device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W) {
+__device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W, int nthr) {
     const int per = n >> R;
-    for (int g = threadIdx.x; g < nt * per; g += blockDim.x) {
+    for (int g = threadIdx.x; g < nt * per; g += nthr) {
         if (g < per)
             row_round_group<P, R, DIF>(t0, logL, lq, g, W);
         else
@@ -671,12 +714,12 @@ __device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, con
 }
 
 template <int P, bool DIF, int RMAX>
-__device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W) {
+__device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W, int nthr) {
     switch (r) {
-        case 1: row_round<P, 1, DIF>(t0, t1, nt, n, logL, lq, W); break;
-        case 2: row_round<P, 2, DIF>(t0, t1, nt, n, logL, lq, W); break;
-        case 3: row_round<P, 3, DIF>(t0, t1, nt, n, logL, lq, W); break;
-        default: row_round<P, (RMAX > 3 ? 4 : 3), DIF>(t0, t1, nt, n, logL, lq, W); break;
+        case 1: row_round<P, 1, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 2: row_round<P, 2, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 3: row_round<P, 3, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        default: row_round<P, (RMAX > 3 ? 4 : 3), DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
     }
 }
 
@@ -687,14 +730,14 @@ __device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, in
 // depend on the lane.  Modes kRowFwdOnly / kRowFwdKeep stop after the
 // forward stages.
 template <int P, int R>
-__device__ void row_fused(const RowArgs& A, const RowLine* s_line, u64* tA, u64* tB, int n, int logL) {
+__device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_line, u64* tA, u64* tB, int n, int logL) {
     constexpr int Q = 1 << R;
     const u64* __restrict__ W = A.tw[P];
     const u64* __restrict__ IW = A.itw[P];
     const bool two = A.mode == kRowMul;
     const bool fwd = A.mode != kRowSqrSpec;
     const bool inv = A.mode == kRowMul || A.mode == kRowSqr || A.mode == kRowMulSpec || A.mode == kRowSqrSpec;
-    for (int g = threadIdx.x; g < (n >> R); g += blockDim.x) {
+    for (int g = threadIdx.x; g < (n >> R); g += A.threads) {
         const int base = g << R;
         u64 x[Q], y[Q];
 #pragma unroll
@@ -721,10 +764,12 @@ __device__ void row_fused(const RowArgs& A, const RowLine* s_line, u64* tA, u64*
             continue;
         }
         if (A.mode == kRowMulSpec) {
+            // an inactive line (never stored) reads line 0's spectrum
             const RowLine& li = s_line[base >> logL];
-            const u64 j0 = li.g0 + (u64)(base & ((1 << logL) - 1));
+            const FMH_GLOBAL u64* rs =
+                gmem(li.active ? li.rs + li.g0 + (u64)(base & ((1 << logL) - 1)) : s_line[0].rs);
 #pragma unroll
-            for (int q = 0; q < Q; q++) x[q] = redc<P>(x[q], li.active ? li.rs[j0 + q] : 0ull);
+            for (int q = 0; q < Q; q++) x[q] = redc<P>(x[q], rs[q]);
         } else {
 #pragma unroll
             for (int q = 0; q < Q; q++) x[q] = redc<P>(x[q], two ? y[q] : x[q]);
@@ -748,7 +793,7 @@ __device__ void row_fused(const RowArgs& A, const RowLine* s_line, u64* tA, u64*
 // operands, more registers; 3: radix-8 rounds, fewer registers and more
 // waves per SIMD).
 template <int P, int RMAX>
-__device__ void row_body(const RowArgs& A, RowLine* s_line) {
+__device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
     extern __shared__ u64 lds[];
     const int logL = A.log_n - A.s0;
     const int logC = A.logC;
@@ -764,51 +809,73 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
         const u64 prod = lambda >> A.s0;
         RowLine li{};
         li.active = prod < A.count ? 1u : 0u;
+        for (int op = 0; op < 2; op++) {
+            li.src[op] = A.tw[P];
+            li.last[op] = 0;
+            li.len[op] = ~0ull;
+        }
         if (li.active) {
             const ProductDesc& D = product(A.items, A.one, (u32)prod);
-            li.a = D.a;
-            li.b = D.b;
-            li.na = D.na;
-            li.nb = D.nb;
             li.ra = D.res + ((u64)P << A.log_n);
-            li.rb = D.res + ((u64)(3 + P) << A.log_n);
             li.rs = D.spec ? D.spec + ((u64)P << A.log_n) : nullptr;
             li.g0 = (lambda & ((1ull << A.s0) - 1)) << logL;
+            for (int op = 0; op < (two ? 2 : 1); op++) {
+                if (spec_in) {
+                    li.src[op] = li.rs;
+                    li.last[op] = ~0ull;
+                } else if (A.from_limbs) {
+                    const u64 len = op ? D.nb : D.na;
+                    li.len[op] = len;
+                    if (len) {
+                        li.src[op] = op ? D.b : D.a;
+                        li.last[op] = len - 1;
+                    }
+                } else {
+                    li.src[op] = D.res + ((u64)(3 * op + P) << A.log_n);
+                    li.last[op] = ~0ull;
+                }
+            }
         }
         s_line[threadIdx.x] = li;
     }
     __syncthreads();
     // Lines at least one workgroup long are walked line by line (the line's
     // description is then uniform); shorter lines element by element.
+    const int nthr = A.threads;
     const int L = 1 << logL;
-    const bool by_line = L >= (int)blockDim.x;
-    auto load = [&](const RowLine& li, u64 g, int e) {
-        u64 va = 0, vb = 0;
-        if (li.active) {
-            if (spec_in) {
-                va = li.rs[g];
-            } else if (A.from_limbs) {
-                va = g < li.na ? reduce_word<P>(li.a[g]) : 0ull;
-                if (two) vb = g < li.nb ? reduce_word<P>(li.b[g]) : 0ull;
-            } else {
-                va = li.ra[g];
-                if (two) vb = li.rb[g];
-            }
-        }
-        tA[row_swz(e)] = va;
-        if (two) tB[row_swz(e)] = vb;
+    const bool by_line = L >= nthr;
+    auto word_ptr = [&](const RowLine& li, int op, u64 g) {
+        const u64 l = li.last[op];
+        return gmem(li.src[op] + (g < l ? g : l));
     };
-    if (by_line) {
-        for (int c = 0; c < C; c++) {
-            const RowLine li = s_line[c];
-            for (int j = threadIdx.x; j < L; j += blockDim.x) load(li, li.g0 + (u64)j, (c << logL) + j);
+    auto word_val = [&](const RowLine& li, int op, u64 g, u64 v) -> u64 {
+        if (!li.active || g >= li.len[op]) return 0;
+        return A.from_limbs && !spec_in ? reduce_word<P>(v) : v;
+    };
+    auto fill = [&](u64* t, int op) {
+        if (by_line) {
+            for (int c = 0; c < C; c++) {
+                const RowLine li = s_line[c];
+                const int e0 = c << logL;
+                batched_copy(
+                    L, nthr, [&](int j) { return *word_ptr(li, op, li.g0 + (u64)j); },
+                    [&](int j, u64 v) { t[row_swz(e0 + j)] = word_val(li, op, li.g0 + (u64)j, v); });
+            }
+        } else {
+            batched_copy(
+                n, nthr,
+                [&](int e) {
+                    const RowLine& li = s_line[e >> logL];
+                    return *word_ptr(li, op, li.g0 + (u64)(e & (L - 1)));
+                },
+                [&](int e, u64 v) {
+                    const RowLine& li = s_line[e >> logL];
+                    t[row_swz(e)] = word_val(li, op, li.g0 + (u64)(e & (L - 1)), v);
+                });
         }
-    } else {
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-            const RowLine& li = s_line[e >> logL];
-            load(li, li.g0 + (u64)(e & (L - 1)), e);
-        }
-    }
+    };
+    fill(tA, 0);
+    if (two) fill(tB, 1);
     __syncthreads();
     // Stage schedule: the last RF forward stages, the pointwise product and
     // the first RF inverse stages run fused in registers; the other stages
@@ -821,7 +888,7 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
         while (top > RF) {
             const int left = top - RF;
             const int r = left % rmax ? left % rmax : rmax;
-            row_round_r<P, true, RMAX>(r, tA, tB, nt, n, logL, top - r, A.tw[P]);
+            row_round_r<P, true, RMAX>(r, tA, tB, nt, n, logL, top - r, A.tw[P], nthr);
             top -= r;
         }
     }
@@ -834,29 +901,36 @@ __device__ void row_body(const RowArgs& A, RowLine* s_line) {
     if (A.mode == kRowFwdOnly || A.mode == kRowFwdKeep) {
         const u64 r2 = A.r2[P];
         const bool mont = A.mode == kRowFwdOnly;
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-            const RowLine& li = s_line[e >> logL];
-            const u64 v = tA[row_swz(e)];
-            if (li.active) li.ra[li.g0 + (e & (L - 1))] = canon<P>(mont ? redc<P>(v, r2) : norm2p<P>(v));
-        }
+        batched_copy(
+            n, nthr, [&](int e) { return tA[row_swz(e)]; },
+            [&](int e, u64 v) {
+                const RowLine& li = s_line[e >> logL];
+                if (li.active)
+                    gmem(li.ra)[li.g0 + (e & (L - 1))] = canon<P>(mont ? redc<P>(v, r2) : norm2p<P>(v));
+            });
         return;
     }
     for (int lq = RF; lq < logL;) {
         const int r = logL - lq < 3 ? logL - lq : 3;
-        row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P]);
+        row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P], nthr);
         lq += r;
     }
     if (by_line) {
         for (int c = 0; c < C; c++) {
             const RowLine li = s_line[c];
             if (!li.active) continue;
-            for (int j = threadIdx.x; j < L; j += blockDim.x) li.ra[li.g0 + j] = tA[row_swz((c << logL) + j)];
+            FMH_GLOBAL u64* out = gmem(li.ra + li.g0);
+            const int e0 = c << logL;
+            batched_copy(
+                L, nthr, [&](int j) { return tA[row_swz(e0 + j)]; }, [&](int j, u64 v) { out[j] = v; });
         }
     } else {
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-            const RowLine& li = s_line[e >> logL];
-            if (li.active) li.ra[li.g0 + (e & (L - 1))] = tA[row_swz(e)];
-        }
+        batched_copy(
+            n, nthr, [&](int e) { return tA[row_swz(e)]; },
+            [&](int e, u64 v) {
+                const RowLine& li = s_line[e >> logL];
+                if (li.active) gmem(li.ra)[li.g0 + (e & (L - 1))] = v;
+            });
     }
 }
 
@@ -921,22 +995,23 @@ __device__ __forceinline__ void crt_load(const ProductDesc& D, u64 k, u64& v0, u
     if (k >= D.na + D.nb - 1) return;
     const u64 N = 1ull << D.log_n;
     if (!D.split_n) {
-        v0 = D.res[k];
-        v1 = D.res[k + N];
-        v2 = D.res[k + 2 * N];
+        const FMH_GLOBAL u64* q = gmem(D.res + k);
+        v0 = q[0];
+        v1 = q[N];
+        v2 = q[2 * N];
         return;
     }
     const u32 m = (u32)D.split_m;
     const u32 i = (u32)k / m;  // k < 2^28
     const u64 r = k - (u64)i * m;
     if (i < D.split_n) {
-        const u64* q = D.res + (u64)i * 3 * N + r;
+        const FMH_GLOBAL u64* q = gmem(D.res + (u64)i * 3 * N + r);
         v0 = norm2p<0>(q[0]);
         v1 = norm2p<1>(q[N]);
         v2 = norm2p<2>(q[2 * N]);
     }
     if (i >= 1 && i - 1 < D.split_n && r + m < N) {
-        const u64* q = D.res + (u64)(i - 1) * 3 * N + r + m;
+        const FMH_GLOBAL u64* q = gmem(D.res + (u64)(i - 1) * 3 * N + r + m);
         v0 += norm2p<0>(q[0]);
         v1 += norm2p<1>(q[N]);
         v2 += norm2p<2>(q[2 * N]);
@@ -1006,8 +1081,27 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     const u64 out_len = D.na + D.nb;
 
     u64 v[E][3];
+    if (!D.split_n) {
+        // all 3E loads in flight at once: indices past the product are
+        // clamped to its last coefficient and their values dropped
+        const u64 kend = D.na + D.nb - 1;
+        const u64 N = 1ull << D.log_n;
+        const FMH_GLOBAL u64* res = gmem(D.res);
 #pragma unroll
-    for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
+        for (int e = 0; e < E; e++) {
+            const u64 k = k0 + (u64)(e * kThreads + tid);
+            const u64 kc = k < kend ? k : kend - 1;
+            v[e][0] = res[kc];
+            v[e][1] = res[kc + N];
+            v[e][2] = res[kc + 2 * N];
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            if (k0 + (u64)(e * kThreads + tid) >= kend) v[e][0] = v[e][1] = v[e][2] = 0;
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
+    }
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
@@ -1049,10 +1143,11 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
 #pragma unroll
     for (int e = 0; e < E; e++) s_w0[tid * E + e] = lo[e];
     __syncthreads();
+    FMH_GLOBAL u64* out = gmem(D.out);
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
-        if (k0 + (u64)i < out_len) D.out[k0 + i] = s_w0[i];
+        if (k0 + (u64)i < out_len) out[k0 + i] = s_w0[i];
     }
 
     u32 agg;
@@ -1121,11 +1216,20 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
         c = map_apply((u32)(packed >> (8 * e)) & 0xFFu, c);
     }
     __syncthreads();
+    // the chunk starts inside the product: k0 < out_len
+    FMH_GLOBAL u64* out = gmem(D.out + k0);
+    const u64 lim = out_len - k0;
+    u64 w[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const u64 i = (u64)(e * kThreads + tid);
+        w[e] = out[i < lim ? i : lim - 1];
+    }
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int i = e * kThreads + tid;
         const u32 ci = s_c[i];
-        if (ci != 0 && k0 + (u64)i < out_len) D.out[k0 + i] += ci;
+        if (ci != 0 && (u64)i < lim) out[i] = w[e] + ci;
     }
 }
 
@@ -1513,6 +1617,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
         C.nops = nops;
+        C.threads = tu.col_threads;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
@@ -1537,6 +1642,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
                                                 "row_fwd_keep"};
         static const int kRowModeOf[5] = {kRowMul, kRowSqr, kRowMulSpec, kRowSqrSpec, kRowFwdKeep};
         R.mode = kRowModeOf[kind];
+        R.threads = tu.row_threads;
         const int logL = log_n - pl.row_s;
         const u64 lines = (u64)count << pl.row_s;
         int logC = std::max(0, std::min(tu.row_tile - logL, 6));  // kMaxRowLines = 64
@@ -1561,6 +1667,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = 0;
         C.nops = 1;
+        C.threads = tu.col_threads;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
@@ -1649,6 +1756,7 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
         C.s1 = pl.col_s[pi + 1];
         C.from_limbs = (pi == 0);
         C.nops = 1;
+        C.threads = tu.col_threads;
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
@@ -1670,6 +1778,7 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     R.s0 = pl.row_s;
     R.from_limbs = (pl.n_col == 0);
     R.mode = mode;
+    R.threads = tu.row_threads;
     const int logL = log_n - pl.row_s;
     const int logC = std::max(0, std::min({pl.row_s, tu.row_tile - logL, 6}));
     R.logC = logC;
