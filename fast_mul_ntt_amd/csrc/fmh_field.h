@@ -162,6 +162,15 @@ FMH_HD void bf_dif(u64& x, u64& y, u64 w) {
     x = s;
 }
 
+// The DIF butterfly with w = 1: (x + y, x - y), outputs in [0, 2p).
+template <int P>
+FMH_HD void bf_dif_one(u64& x, u64& y) {
+    constexpr u64 p2 = 2 * Prime<P>::p;
+    const u64 s = add2p<P>(x, y);
+    y = norm2p<P>(x + p2 - y);
+    x = s;
+}
+
 // Cooley-Tukey (DIT) butterfly, x, y in [0, 4p): (x + y w, x - y w), outputs
 // in [0, 4p).
 template <int P>
@@ -169,6 +178,16 @@ FMH_HD void bf_dit(u64& x, u64& y, u64 w) {
     constexpr u64 p2 = 2 * Prime<P>::p;
     const u64 xr = norm2p<P>(x);
     const u64 t = redc<P>(y, w);
+    x = xr + t;
+    y = xr + p2 - t;
+}
+
+// The DIT butterfly with w = 1, same ranges.
+template <int P>
+FMH_HD void bf_dit_one(u64& x, u64& y) {
+    constexpr u64 p2 = 2 * Prime<P>::p;
+    const u64 xr = norm2p<P>(x);
+    const u64 t = norm2p<P>(y);
     x = xr + t;
     y = xr + p2 - t;
 }

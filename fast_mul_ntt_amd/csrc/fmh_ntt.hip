@@ -752,6 +752,11 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     if (q & hq) continue;
+                    if ((q & (hq - 1)) == 0) {  // twiddle w_{2 hq}^0 = 1
+                        bf_dif_one<P>(x[q], x[q + hq]);
+                        if (two) bf_dif_one<P>(y[q], y[q + hq]);
+                        continue;
+                    }
                     const u64 w = W[hq + (q & (hq - 1))];
                     bf_dif<P>(x[q], x[q + hq], w);
                     if (two) bf_dif<P>(y[q], y[q + hq], w);
@@ -780,7 +785,10 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 if (q & hq) continue;
-                bf_dit<P>(x[q], x[q + hq], IW[hq + (q & (hq - 1))]);
+                if ((q & (hq - 1)) == 0)
+                    bf_dit_one<P>(x[q], x[q + hq]);
+                else
+                    bf_dit<P>(x[q], x[q + hq], IW[hq + (q & (hq - 1))]);
             }
         }
 #pragma unroll

@@ -87,6 +87,17 @@ static void check_prime(long iters) {
         const u64 t = (u64)((u128)(ey % p) * rinv_w % p);
         CHECK(dx % p == (ex % p + t) % p);
         CHECK(dy % p == (ex % p + p - t) % p);
+        // the w = 1 butterflies: same values mod p as the plain ones with the
+        // Montgomery form of 1, same ranges
+        const u64 one_m = (u64)(((u128)1 << 64) % p);
+        u64 ux = next64() % p2, uy = next64() % p2, vx = ux, vy = uy;
+        bf_dif_one<P>(ux, uy);
+        bf_dif<P>(vx, vy, one_m);
+        CHECK(ux < p2 && uy < p2 && ux % p == vx % p && uy % p == vy % p);
+        u64 sx = next64() % p4, sy = next64() % p4, tx = sx, ty = sy;
+        bf_dit_one<P>(sx, sy);
+        bf_dit<P>(tx, ty, one_m);
+        CHECK(sx < p4 && sy < p4 && sx % p == tx % p && sy % p == ty % p);
         const u64 q = next64() % p4;
         CHECK(norm2p<P>(q) == (q >= p2 ? q - p2 : q));
         const u64 c2 = next64() % p2;
