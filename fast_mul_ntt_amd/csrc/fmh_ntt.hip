@@ -332,7 +332,9 @@ __device__ __forceinline__ FMH_GLOBAL T* gmem(T* p) {
 // its value dropped), which keeps them in one basic block.
 constexpr int kBatch = 8;
 
-// For e in [0, n): put(e, get(min(e', n - 1))) ... with get() a plain load.
+// put(e, get(e)) for every tile word e in [0, n) of this thread (e = tid +
+// k nthr).  get() must be a plain load that is valid for any e < n: the
+// step's loads all run first (past n, get(n - 1) is loaded and dropped).
 template <class Get, class Put>
 __device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) {
     for (int e0 = threadIdx.x; e0 < n; e0 += kBatch * nthr) {
