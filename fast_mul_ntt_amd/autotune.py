@@ -1,6 +1,7 @@
 """Autotuner of the NTT pass geometry, one entry per transform length.
 
     python -m fast_mul_ntt_amd.autotune [--min-log 10] [--max-log 25] [--out PATH]
+    python -m fast_mul_ntt_amd.autotune --batched-only [--out PATH]
 
 For each size class 2^L it times a balanced product of two 2^(L-1)-limb
 operands (device-resident, seeded operands) under candidate geometries and
@@ -8,13 +9,14 @@ keeps the fastest; for classes up to 2^17 it also tunes the geometry of
 batched launches, timing one batch of many such products (the table's
 "b L spec" lines, used when a launch carries several products of a class).
 The search is coordinate descent over the parameters of a geometry spec,
-starting from the defaults:
+starting from the defaults (batched classes: from the class's single-product
+entry):
 
     row_maxl     points per row-pass line          2^8 .. 2^12
     row_tile     points per row-pass tile          2^10 .. 2^13
     col_maxl     points per column-pass line       2^7 .. 2^11
-    col_logc     adjacent lines per column tile    4 .. 16
-    col_threads  threads per column workgroup      256 .. 1024
+    col_logc     adjacent lines per column tile    4 .. 32
+    col_threads  threads per column workgroup      64 .. 1024
     row_threads  threads per row workgroup         256 .. 1024
     row_radix    stages per row-pass LDS round     3 or 4
 
@@ -47,8 +49,8 @@ CANDIDATES = {
     "row_maxl": [8, 9, 10, 11, 12],
     "row_tile": [10, 11, 12, 13],
     "col_maxl": [7, 8, 9, 10, 11],
-    "col_logc": [2, 3, 4],
-    "col_threads": [256, 512, 1024],
+    "col_logc": [2, 3, 4, 5],
+    "col_threads": [64, 128, 256, 512, 1024],
     "row_threads": [256, 512, 1024],
     "row_radix": [3, 4],
 }
@@ -140,8 +142,9 @@ class Case:
 def tune(L: int, stream: fm.Stream, reps: int, rounds: int, count: int = 1) -> tuple[str, float, float]:
     case = Case(L, stream, count)
     try:
+        # batched classes start from the class's single-product geometry
         fm.set_tuning(L, "", case.batched)
-        base = parse(fm.get_tuning(L, case.batched))
+        base = parse(fm.get_tuning(L, False) if case.batched else fm.get_tuning(L, case.batched))
         case.check(fmt(base))  # the reference product
         t = case.burst(fmt(base))
         case.per = max(1, int(3e-3 / t))
@@ -177,6 +180,8 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="tune only the batched classes; keep the single-product lines of --out")
     args = ap.parse_args()
     if fm.lib().fmh_init(0) != 0:
         raise fm.FmhError(fm.lib().fmh_last_error().decode())
@@ -184,10 +189,14 @@ def main() -> int:
     lines = ["# NTT pass geometry per transform length 2^L (written by fast_mul_ntt_amd/autotune.py)",
              "# L spec      # us tuned / us default, one balanced 2^(L-1) x 2^(L-1) multiply",
              "# b L spec    # the same for one batched launch of max(8, 2^22 / 2^L) such products"]
-    for L in range(args.min_log, args.max_log + 1):
-        spec, us, d_us = tune(L, stream, args.reps, args.rounds)
-        lines.append(f"{L} {spec}  # {us:.1f} / {d_us:.1f}")
-        print(lines[-1], flush=True)
+    if args.batched_only:
+        with open(args.out) as f:
+            lines += [ln.rstrip("\n") for ln in f if ln[:1].isdigit()]
+    else:
+        for L in range(args.min_log, args.max_log + 1):
+            spec, us, d_us = tune(L, stream, args.reps, args.rounds)
+            lines.append(f"{L} {spec}  # {us:.1f} / {d_us:.1f}")
+            print(lines[-1], flush=True)
     for L in range(args.min_log, args.max_batch_log + 1):
         count = max(8, (1 << 22) >> L)
         spec, us, d_us = tune(L, stream, args.reps, args.rounds, count)
