@@ -349,61 +349,45 @@ __device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) 
 
 // ---------------------------------------------------------------- LDS tiles
 //
-// A tile holds C = 2^logC lines of L = 2^logL words.
-//   MAP 0 (column tile): word (c, j) at t[(j << logC) + c]; line c has stride
-//          offset b = b0 + c within the global stride B = 2^logB.
-//   MAP 1 (row tile):    word (c, j) at t[(c << logL) + j]; stride B = 1.
-// For a stage of local half-span hl the global half-span is H = hl * B and
-// the twiddle of the lower element at local j is tw[H + (j mod hl) B + b].
+// A column tile holds C = 2^logC lines of L = 2^logL words: word (c, j) at
+// t[(j << logC) + c], so every tile row is C adjacent words.  Line c has
+// stride offset b = b0 + c within the global stride B = 2^logB.  For a stage
+// of local half-span hl the global half-span is H = hl B, and the twiddle of
+// the lower element at local j is W[H + (j mod hl) B + b] = W[((hl + j mod hl)
+// << logB) + b0 + c].  Consecutive threads take consecutive c, then
+// consecutive j: a wave reads whole tile rows.
 
-template <int P, int MAP, int NT>
-__device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__ t1, int logL,
-                                         int logC, int logB, u32 b0,
+// Radix-2^2 DIF rounds over the stages hl = L/2 .. 1 (one radix-2 stage last
+// when logL is odd).
+template <int P>
+__device__ __forceinline__ void tile_dif(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
                                          const u64* __restrict__ W, int nthr) {
+    const int C = 1 << logC;
     int rem = logL;
     while (rem >= 2) {
         const int lq = rem - 2;
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            int c, u, blk;
-            if (MAP == 0) {
-                c = beta & ((1 << logC) - 1);
-                const int r = beta >> logC;
-                u = r & (hl - 1);
-                blk = r >> lq;
-            } else {
-                u = beta & (hl - 1);
-                const int r = beta >> lq;
-                blk = r & ((1 << (logL - 2 - lq)) - 1);
-                c = r >> (logL - 2 - lq);
-            }
-            const int j0 = (blk << (lq + 2)) + u;
-            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const int c = beta & (C - 1);
+            const int r = beta >> logC;
+            const int u = r & (hl - 1);
+            const int j0 = ((r >> lq) << (lq + 2)) + u;
+            const u32 bb = b0 + (u32)c;
             const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
             const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
             const u64 wc = W[((u32)(hl + u) << logB) + bb];
-            int i0, st;
-            if (MAP == 0) {
-                i0 = (j0 << logC) + c;
-                st = hl << logC;
-            } else {
-                i0 = (c << logL) + j0;
-                st = hl;
-            }
-#pragma unroll
-            for (int q = 0; q < NT; q++) {
-                u64* t = (q == 0) ? t0 : t1;
-                u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
-                bf_dif<P>(x0, x2, wa);
-                bf_dif<P>(x1, x3, wb);
-                bf_dif<P>(x0, x1, wc);
-                bf_dif<P>(x2, x3, wc);
-                t[i0] = x0;
-                t[i0 + st] = x1;
-                t[i0 + 2 * st] = x2;
-                t[i0 + 3 * st] = x3;
-            }
+            const int i0 = (j0 << logC) + c;
+            const int st = hl << logC;
+            u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
+            bf_dif<P>(x0, x2, wa);
+            bf_dif<P>(x1, x3, wb);
+            bf_dif<P>(x0, x1, wc);
+            bf_dif<P>(x2, x3, wc);
+            t[i0] = x0;
+            t[i0 + st] = x1;
+            t[i0 + 2 * st] = x2;
+            t[i0 + 3 * st] = x3;
         }
         __syncthreads();
         rem -= 2;
@@ -411,65 +395,34 @@ __device__ __forceinline__ void tile_dif(u64* __restrict__ t0, u64* __restrict__
     if (rem == 1) {
         const int nbf = 1 << (logL - 1 + logC);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            int c, blk;
-            if (MAP == 0) {
-                c = beta & ((1 << logC) - 1);
-                blk = beta >> logC;
-            } else {
-                blk = beta & ((1 << (logL - 1)) - 1);
-                c = beta >> (logL - 1);
-            }
-            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
-            const u64 w = W[(1u << logB) + bb];
-            int i0, st;
-            if (MAP == 0) {
-                i0 = ((2 * blk) << logC) + c;
-                st = 1 << logC;
-            } else {
-                i0 = (c << logL) + 2 * blk;
-                st = 1;
-            }
-#pragma unroll
-            for (int q = 0; q < NT; q++) {
-                u64* t = (q == 0) ? t0 : t1;
-                u64 x0 = t[i0], x1 = t[i0 + st];
-                bf_dif<P>(x0, x1, w);
-                t[i0] = x0;
-                t[i0 + st] = x1;
-            }
+            const int c = beta & (C - 1);
+            const u64 w = W[(1u << logB) + b0 + (u32)c];
+            const int i0 = ((2 * (beta >> logC)) << logC) + c;
+            u64 x0 = t[i0], x1 = t[i0 + C];
+            bf_dif<P>(x0, x1, w);
+            t[i0] = x0;
+            t[i0 + C] = x1;
         }
         __syncthreads();
     }
 }
 
-template <int P, int MAP>
+// The inverse: one radix-2 stage first when logL is odd, then radix-2^2 DIT
+// rounds up to hl = L/2.
+template <int P>
 __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
                                          const u64* __restrict__ W, int nthr) {
+    const int C = 1 << logC;
     if (logL & 1) {
         const int nbf = 1 << (logL - 1 + logC);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            int c, blk;
-            if (MAP == 0) {
-                c = beta & ((1 << logC) - 1);
-                blk = beta >> logC;
-            } else {
-                blk = beta & ((1 << (logL - 1)) - 1);
-                c = beta >> (logL - 1);
-            }
-            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
-            const u64 w = W[(1u << logB) + bb];
-            int i0, st;
-            if (MAP == 0) {
-                i0 = ((2 * blk) << logC) + c;
-                st = 1 << logC;
-            } else {
-                i0 = (c << logL) + 2 * blk;
-                st = 1;
-            }
-            u64 x0 = t[i0], x1 = t[i0 + st];
+            const int c = beta & (C - 1);
+            const u64 w = W[(1u << logB) + b0 + (u32)c];
+            const int i0 = ((2 * (beta >> logC)) << logC) + c;
+            u64 x0 = t[i0], x1 = t[i0 + C];
             bf_dit<P>(x0, x1, w);
             t[i0] = x0;
-            t[i0 + st] = x1;
+            t[i0 + C] = x1;
         }
         __syncthreads();
     }
@@ -477,31 +430,16 @@ __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            int c, u, blk;
-            if (MAP == 0) {
-                c = beta & ((1 << logC) - 1);
-                const int r = beta >> logC;
-                u = r & (hl - 1);
-                blk = r >> lq;
-            } else {
-                u = beta & (hl - 1);
-                const int r = beta >> lq;
-                blk = r & ((1 << (logL - 2 - lq)) - 1);
-                c = r >> (logL - 2 - lq);
-            }
-            const int j0 = (blk << (lq + 2)) + u;
-            const u32 bb = (MAP == 0) ? b0 + (u32)c : 0u;
+            const int c = beta & (C - 1);
+            const int r = beta >> logC;
+            const int u = r & (hl - 1);
+            const int j0 = ((r >> lq) << (lq + 2)) + u;
+            const u32 bb = b0 + (u32)c;
             const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
             const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
             const u64 wc = W[((u32)(hl + u) << logB) + bb];
-            int i0, st;
-            if (MAP == 0) {
-                i0 = (j0 << logC) + c;
-                st = hl << logC;
-            } else {
-                i0 = (c << logL) + j0;
-                st = hl;
-            }
+            const int i0 = (j0 << logC) + c;
+            const int st = hl << logC;
             u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
             bf_dit<P>(x0, x1, wc);
             bf_dit<P>(x2, x3, wc);
@@ -589,9 +527,9 @@ __device__ void col_body(const ColArgs& A) {
     }
     __syncthreads();
     if (FWD)
-        tile_dif<P, 0, 1>(lds, nullptr, logL, logC, logB, b0, A.tw[P], nthr);
+        tile_dif<P>(lds, logL, logC, logB, b0, A.tw[P], nthr);
     else
-        tile_dit<P, 0>(lds, logL, logC, logB, b0, A.tw[P], nthr);
+        tile_dit<P>(lds, logL, logC, logB, b0, A.tw[P], nthr);
     batched_copy(n, nthr, [&](int e) { return lds[e]; }, [&](int e, u64 v) { res[at(e)] = v; });
 }
 
