@@ -200,6 +200,7 @@ std::map<hipStream_t, Staging> g_stage;
 // Enqueue a set of products on stream s; callers hold g_mu.
 int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
     Job job;
+    if (!products_valid(in, count)) return set_err("NULL device pointer for a non-empty operand or output");
     const int rc = job_build(job, in, count);
     if (rc == -2) return set_err("product longer than the spectrum's transform length");
     if (rc) return set_err("product longer than fmh_max_ntt_len()");
@@ -350,6 +351,7 @@ void abs_sqr64(const u64* in1, u64 len1, u64* out) {
 
 int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
                    fmh_stream_t stream) {
+    if (len2 && !d_in2) return set_err("NULL device pointer for a non-empty operand");
     std::lock_guard<std::mutex> lk(g_mu);
     DevCtx* c = nullptr;
     if (ctx_get(&c)) return -1;
@@ -368,6 +370,7 @@ int fmh_forward_spectrum_device(int prime, const u64* d_in, u64 len, u64 ntt_len
     if (prime < 0 || prime > 2) return set_err("prime index must be 0, 1 or 2");
     if (ntt_len == 0 || (ntt_len & (ntt_len - 1)) || ntt_len < len)
         return set_err("ntt_len must be a power of two >= len");
+    if ((len && !d_in) || !d_out) return set_err("NULL device pointer");
     const int log_n = ceil_log2(ntt_len);
     if (log_n > kMaxLogN) return set_err("ntt_len above fmh_max_ntt_len()");
     std::lock_guard<std::mutex> lk(g_mu);
@@ -410,6 +413,7 @@ int fmh_spectrum_create(const u64* d_in, u64 len, u64 ntt_len, fmh_stream_t stre
     if (!out) return set_err("NULL argument");
     *out = nullptr;
     if (len == 0) return set_err("an empty operand has no spectrum");
+    if (!d_in) return set_err("NULL operand");
     if (ntt_len == 0) ntt_len = 1ull << ceil_log2(2 * len - 1);
     if ((ntt_len & (ntt_len - 1)) || ntt_len < len) return set_err("ntt_len must be a power of two >= len");
     const int log_n = ceil_log2(ntt_len);
@@ -510,6 +514,8 @@ int fmh_batch_create(const fmh_product* items, u64 count, fmh_batch** out) {
         delete b;
         return e == hipSuccess ? set_err(what) : set_err(what, e);
     };
+    if (!products_valid(in.data(), in.size()))
+        return fail("NULL device pointer for a non-empty operand or output", hipSuccess);
     if (job_build(b->job, in.data(), in.size())) return fail("product longer than fmh_max_ntt_len()", hipSuccess);
     if (ensure_twiddles(*c, b->job.max_log_n)) {
         delete b;

@@ -168,6 +168,9 @@ void prepare_kernels();
 // Fills the job; returns 0, -1 if a product is longer than 2^kMaxLogN, or -2
 // if a product does not fit the transform length of its resident spectrum.
 int job_build(Job& j, const ProductIn* in, size_t count);
+// False if a non-empty operand or output of the products is a null pointer
+// (which the kernels would dereference).
+bool products_valid(const ProductIn* in, size_t count);
 size_t job_ws_bytes(const Job& j);
 void job_bind(Job& j, void* ws);
 // Descriptors that kernels read from device memory (jobs of more than one

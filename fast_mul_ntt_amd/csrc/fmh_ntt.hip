@@ -1363,6 +1363,19 @@ static int split_plan(u64 n1, u64 n2, int L, u64* m_out, u32* pieces) {
     return best_k;
 }
 
+bool products_valid(const ProductIn* in, size_t count) {
+    for (size_t i = 0; i < count; i++) {
+        const ProductIn& p = in[i];
+        // the operand that is transformed (for a spectrum product: the other
+        // one; a null b means a square), and the output (the lengths as in
+        // job_build)
+        if (p.na && !p.a) return false;
+        const u64 out_len = p.spec ? (p.a ? p.na + p.nb : 2 * p.nb) : (p.b ? p.na + p.nb : 2 * p.na);
+        if (out_len && !p.out) return false;
+    }
+    return true;
+}
+
 int job_build(Job& j, const ProductIn* in, size_t count) {
     j = Job();
     std::vector<ProductDesc> items;

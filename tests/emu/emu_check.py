@@ -88,6 +88,20 @@ print("unbalanced ok", flush=True)
 ones = np.full(300, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
 check(np.array_equal(mul(ones, ones[:200]), orc.mul(ones, ones[:200])), "all ones")
 
+# null pointers with nonzero lengths are refused before any launch
+L.fmh_mul_device.argtypes = [vp, u, vp, u, vp, vp]
+L.fmh_last_error.restype = ctypes.c_char_p
+buf = np.zeros(64, np.uint64)
+check(L.fmh_mul_device(None, 8, buf.ctypes.data, 8, buf.ctypes.data + 256, None) != 0, "null operand")
+check(b"NULL" in L.fmh_last_error(), "null operand message")
+check(L.fmh_mul_device(buf.ctypes.data, 8, buf.ctypes.data, 8, None, None) != 0, "null output")
+check(L.fmh_forward_spectrum_device(0, None, 8, 16, buf.ctypes.data, None) != 0, "null spectrum input")
+h0 = vp()
+check(L.fmh_spectrum_create(None, 8, 0, None, ctypes.byref(h0)) != 0, "null spectrum operand")
+buf[:] = 7
+check(L.fmh_mul_device(None, 0, buf.ctypes.data, 8, buf.ctypes.data + 256, None) == 0, "empty null operand")
+check(not buf[32:40].any(), "empty product is zero")
+
 # forward spectra (device pointers are host pointers here)
 for log_n in [0, 3, 13]:
     n = 1 << log_n

@@ -360,6 +360,24 @@ def test_largest_transform_and_refusal_beyond(fm, orc):
         assert rc != 0 and b"fmh_max_ntt_len" in fm.lib().fmh_last_error()
 
 
+def test_null_device_pointers_are_refused(fm):
+    """A null operand or output with a nonzero length is an error returned
+    before any launch (a kernel would fault on it); empty operands may be
+    null."""
+    with fm.DeviceBuffer(64) as d, fm.Stream() as s:
+        lib = fm.lib()
+        assert lib.fmh_mul_device(None, 8, d.ptr, 8, d.ptr + 256, s.handle) != 0
+        assert b"NULL" in lib.fmh_last_error()
+        assert lib.fmh_mul_device(d.ptr, 8, None, 8, d.ptr + 256, s.handle) != 0
+        assert lib.fmh_sqr_device(d.ptr, 8, None, s.handle) != 0
+        assert b"NULL" in lib.fmh_last_error()
+        assert lib.fmh_forward_spectrum_device(0, None, 8, 16, d.ptr, s.handle) != 0
+        # zero-length operands: the all-zero product, nothing is read
+        assert lib.fmh_mul_device(None, 0, d.ptr, 8, d.ptr + 256, s.handle) == 0
+        s.synchronize()
+        assert not d.download(64)[32:40].any()
+
+
 def _primes61(count, seed):
     import random
     rng = random.Random(seed)
