@@ -36,7 +36,8 @@ void abs_sqr64(const u64* in1, u64 len1, u64* out);
  * `stream` (a hipStream_t; NULL is the null stream) and returns before it
  * completes.  Scratch memory is owned by the library, one workspace per
  * stream: calls on one stream run in order, calls on different streams do
- * not share scratch.  `out` must not alias an input.                       */
+ * not share scratch.  `out` must not alias an input.  A null pointer for a
+ * non-empty operand or output is an error returned before anything runs.   */
 typedef void* fmh_stream_t;
 
 int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
