@@ -17,7 +17,7 @@ namespace fmh {
 constexpr int kMaxLogN = 28;
 
 constexpr int kThreads = 256;         // threads per workgroup, CRT and carry kernels
-constexpr int kScanThreads = 256;     // threads of the chunk scan (one workgroup per product)
+constexpr int kScanThreads = 512;     // threads of the chunk scan (one workgroup per product)
 constexpr int kMaxPassThreads = 1024; // launch bound of the NTT pass kernels
 constexpr int kCarryPerThread = 4;    // limbs per thread in the carry scan
 constexpr int kCarryChunk = kThreads * kCarryPerThread;

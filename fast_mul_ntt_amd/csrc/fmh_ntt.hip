@@ -900,8 +900,10 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
 // s_k = c_k[0] + c_{k-1}[1] + c_{k-2}[2] < 3 * 2^64 and carry_k <= 2
 // (c_k < 2^184).  The carry out of limb k is a map f_k: {0..3} -> {0..3},
 // f_k(c) = hi(s_k) + [lo(s_k) + c >= 2^64]; maps compose associatively, so
-// the serial loop of src/fast_mul.c:378-389 becomes a scan over maps.  A map
-// is packed as four 2-bit values, m(c) = (m >> 2c) & 3.
+// the serial loop of src/fast_mul.c:378-389 becomes a scan over maps.  In
+// registers a map is four bytes, byte c = m(c), so that g after f is one byte
+// permute (v_perm_b32 selects byte f(c) of g for byte c); in fmap[] a limb's
+// map is packed into one byte as four 2-bit values.
 //
 // Three kernels, no inter-workgroup waiting and no contended atomics:
 //   k_crt_local   one chunk of limbs per workgroup: CRT, s_k, writes
@@ -921,16 +923,23 @@ struct CarryArgs {
 };
 
 __device__ __forceinline__ u32 map_compose(u32 g, u32 f) {  // g after f
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(g, g, f);
+#else
     u32 r = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const u32 fc = (f >> (2 * c)) & 3u;
-        r |= ((g >> (2 * fc)) & 3u) << (2 * c);
-    }
+    for (int c = 0; c < 4; c++) r |= ((g >> (8 * ((f >> (8 * c)) & 3u))) & 0xFFu) << (8 * c);
     return r;
+#endif
 }
-__device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (2 * c)) & 3u; }
-constexpr u32 kMapId = 0xE4u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
+__device__ __forceinline__ u32 map_apply(u32 m, u32 c) { return (m >> (8 * c)) & 0xFFu; }
+constexpr u32 kMapId = 0x03020100u;  // 0 -> 0, 1 -> 1, 2 -> 2, 3 -> 3
+// fmap[] byte <-> register form
+__device__ __forceinline__ u32 map_pack(u32 m) {
+    return (m & 3u) | ((m >> 6) & 0xCu) | ((m >> 12) & 0x30u) | ((m >> 18) & 0xC0u);
+}
+__device__ __forceinline__ u32 map_unpack(u32 b) {
+    return (b & 3u) | ((b & 0xCu) << 6) | ((b & 0x30u) << 12) | ((b & 0xC0u) << 18);
+}
 
 // Residues of coefficient k (zero beyond the convolution).  For a split
 // product, coefficient k sums those of piece i = k / m at offset k - i m and
@@ -1077,12 +1086,10 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         const u128 s = (u128)s_w0[i] + s_w1[i + 1] + s_w2[i];
         const u64 l = (u64)s;
         const u32 h = (u32)(s >> 64);
-        const u32 f0 = h;
-        const u32 f1 = h + (l == ~0ull ? 1u : 0u);
-        const u32 f2 = h + (l >= ~0ull - 1 ? 1u : 0u);
-        const u32 f3 = h + (l >= ~0ull - 2 ? 1u : 0u);
-        const u32 f = f0 | (f1 << 2) | (f2 << 4) | (f3 << 6);
-        packed |= (u64)f << (8 * e);
+        // byte c: h + [l + c >= 2^64]
+        const u32 f = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
+                      ((l >= ~0ull - 2 ? 1u : 0u) << 24);
+        packed |= (u64)map_pack(f) << (8 * e);
         F = map_compose(f, F);
         lo[e] = l;
     }
@@ -1108,7 +1115,7 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
 // tile, thread t owns a contiguous segment, and the carry out of each tile
 // enters the next.
 #ifndef FMH_SCAN_TILE
-#define FMH_SCAN_TILE 4096  // the CPU emulation builds with a small tile to reach the multi-tile path
+#define FMH_SCAN_TILE 8192  // the CPU emulation builds with a small tile to reach the multi-tile path
 #endif
 constexpr int kScanTile = FMH_SCAN_TILE;
 
@@ -1118,14 +1125,15 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(CarryArgs A) {
     const int tid = threadIdx.x;
     const ProductDesc& D = product(A.items, A.one, blockIdx.x);
     const u32 n = D.nchunks;
-    const u32* maps = A.chunk_map + D.chunk0;
-    u32* cin = A.chunk_cin + D.chunk0;
+    const FMH_GLOBAL u32* maps = gmem(A.chunk_map + D.chunk0);
+    FMH_GLOBAL u32* cin = gmem(A.chunk_cin + D.chunk0);
     u32 carry = 0;  // carry into the current tile
     for (u32 base = 0; base < n; base += kScanTile) {
         const u32 cnt = min(n - base, (u32)kScanTile);
-        for (u32 i = tid; i < cnt; i += blockDim.x) s_m[i] = maps[base + i];
+        batched_copy(
+            (int)cnt, kScanThreads, [&](int i) { return (u64)maps[base + i]; }, [&](int i, u64 v) { s_m[i] = (u32)v; });
         __syncthreads();
-        const u32 per = (cnt + blockDim.x - 1) / blockDim.x;
+        const u32 per = (cnt + kScanThreads - 1) / kScanThreads;
         const u32 beg = min(cnt, (u32)tid * per), end = min(cnt, beg + per);
         u32 seg = kMapId;
         for (u32 i = beg; i < end; i++) seg = map_compose(s_m[i], seg);
@@ -1154,14 +1162,14 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     const u32 packed = *(const u32*)(A.fmap + tile * CH + (u64)tid * E);
     u32 F = kMapId;
 #pragma unroll
-    for (int e = 0; e < E; e++) F = map_compose((u32)(packed >> (8 * e)) & 0xFFu, F);
+    for (int e = 0; e < E; e++) F = map_compose(map_unpack((packed >> (8 * e)) & 0xFFu), F);
     u32 agg;
     const u32 excl = block_excl_scan_maps(F, s_wave, &agg);
     u32 c = map_apply(excl, A.chunk_cin[tile]);
 #pragma unroll
     for (int e = 0; e < E; e++) {
         s_c[tid * E + e] = (u8)c;
-        c = map_apply((u32)(packed >> (8 * e)) & 0xFFu, c);
+        c = map_apply(map_unpack((packed >> (8 * e)) & 0xFFu), c);
     }
     __syncthreads();
     // the chunk starts inside the product: k0 < out_len

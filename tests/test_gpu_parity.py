@@ -252,6 +252,34 @@ def test_batch_mixed_sizes_vs_oracle(fm, orc):
             d.free()
 
 
+def test_batched_table_geometry_per_class(fm, orc):
+    """Batches of 5 multiplies and 1 square in each class 2^10 .. 2^17 run
+    under the table's batched geometry (at least 4 products of a class):
+    every product equals the oracle's."""
+    for L in range(10, 18):
+        n = 1 << (L - 1)
+        ops = [(orc.splitmix(L * 100 + i, n), orc.splitmix(L * 100 + 50 + i, n - i)) for i in range(5)]
+        sq = orc.splitmix(L * 100 + 99, n)
+        bufs, items = [], []
+        for a, b in ops:
+            da, db, do = fm.DeviceBuffer.from_host(a), fm.DeviceBuffer.from_host(b), fm.DeviceBuffer(a.size + b.size)
+            bufs += [da, db, do]
+            items.append((da, a.size, db, b.size, do))
+        dsq, dso = fm.DeviceBuffer.from_host(sq), fm.DeviceBuffer(2 * n)
+        bufs += [dsq, dso]
+        items.append((dsq, n, None, 0, dso))
+        try:
+            with fm.Stream() as s:
+                fm.mul_batch_device(items, s)
+                s.synchronize()
+            for (a, b), it in zip(ops, items):
+                assert np.array_equal(it[4].download(a.size + b.size), orc.mul(a, b)), L
+            assert np.array_equal(dso.download(), orc.sqr(sq)), L
+        finally:
+            for d in bufs:
+                d.free()
+
+
 def test_batch_graph_replays(fm, orc):
     """A planned batch replays its captured graph: new operand values in the
     same buffers give new, correct products on every replay."""
