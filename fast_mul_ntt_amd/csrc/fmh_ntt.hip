@@ -1038,9 +1038,13 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     const u64 out_len = D.na + D.nb;
 
     u64 v[E][3];
+    // threads 0 and 1 also take coefficient k0 - 2 + tid, whose high words
+    // reach into the chunk's first limbs
+    const bool before = tid < 2 && k0 >= (u64)(2 - tid);
+    u64 x[3] = {0, 0, 0};
     if (!D.split_n) {
-        // all 3E loads in flight at once: indices past the product are
-        // clamped to its last coefficient and their values dropped
+        // all 3E + 3 loads in flight at once: indices outside the product
+        // are clamped to its last coefficient and their values dropped
         const u64 kend = D.na + D.nb - 1;
         const u64 N = 1ull << D.log_n;
         const FMH_GLOBAL u64* res = gmem(D.res);
@@ -1052,10 +1056,16 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
             v[e][1] = res[kc + N];
             v[e][2] = res[kc + 2 * N];
         }
+        const u64 kx = before ? k0 - 2 + tid : kend - 1;
+        x[0] = res[kx];
+        x[1] = res[kx + N];
+        x[2] = res[kx + 2 * N];
 #pragma unroll
         for (int e = 0; e < E; e++)
             if (k0 + (u64)(e * kThreads + tid) >= kend) v[e][0] = v[e][1] = v[e][2] = 0;
+        if (!before) x[0] = x[1] = x[2] = 0;
     } else {
+        if (before) crt_load(D, k0 - 2 + tid, x[0], x[1], x[2]);
 #pragma unroll
         for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
     }
@@ -1069,9 +1079,8 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         s_w2[i + 2] = w2;
     }
     if (tid < 2) {
-        u64 v0 = 0, v1 = 0, v2 = 0, w0, w1, w2;
-        if (k0 >= (u64)(2 - tid)) crt_load(D, k0 - 2 + tid, v0, v1, v2);
-        crt_coef(D.crt, v0, v1, v2, w0, w1, w2);
+        u64 w0, w1, w2;
+        crt_coef(D.crt, x[0], x[1], x[2], w0, w1, w2);
         s_w1[tid] = w1;
         s_w2[tid] = w2;
     }
