@@ -907,17 +907,17 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
 //
 // Three kernels, no inter-workgroup waiting and no contended atomics:
 //   k_crt_local   one chunk of limbs per workgroup: CRT, s_k, writes
-//                 lo(s_k) into out[], the limb maps into fmap[] and the chunk's
-//                 composed map into chunk_map[]
+//                 lo(s_k) into out[], for each limb the map from the chunk's
+//                 carry-in to the limb's carry-in into fmap[], and the
+//                 chunk's composed map into chunk_map[]
 //   k_chunk_scan  one workgroup: the carry into every chunk, chunk_cin[]
-//   k_carry_apply rescans the chunk's limb maps from its carry-in and adds
-//                 the carries into out[].
+//   k_carry_apply adds fmap[k](chunk_cin) into each limb of out[].
 
 struct CarryArgs {
     const ProductDesc* items;
     ProductDesc one;
     const u32* chunk_item;  // crt entry of each chunk (nullptr: entry 0)
-    u8* fmap;               // one map per limb, kCarryChunk per chunk
+    u8* fmap;               // per limb: chunk carry-in -> limb carry-in
     u32* chunk_map;         // composed map of each chunk
     u32* chunk_cin;         // carry into each chunk
 };
@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
     __syncthreads();
 
     u32 F = kMapId;
-    u64 packed = 0;
+    u32 f[E];
     u64 lo[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -1096,13 +1096,11 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         const u64 l = (u64)s;
         const u32 h = (u32)(s >> 64);
         // byte c: h + [l + c >= 2^64]
-        const u32 f = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
-                      ((l >= ~0ull - 2 ? 1u : 0u) << 24);
-        packed |= (u64)map_pack(f) << (8 * e);
-        F = map_compose(f, F);
+        f[e] = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
+               ((l >= ~0ull - 2 ? 1u : 0u) << 24);
+        F = map_compose(f[e], F);
         lo[e] = l;
     }
-    *(u32*)(A.fmap + tile * CH + (u64)tid * E) = (u32)packed;
     __syncthreads();  // all reads of s_w0 done
 #pragma unroll
     for (int e = 0; e < E; e++) s_w0[tid * E + e] = lo[e];
@@ -1114,8 +1112,16 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         if (k0 + (u64)i < out_len) out[k0 + i] = s_w0[i];
     }
 
+    // carry-in of limb tid E + e as a map of the chunk's carry-in
     u32 agg;
-    (void)block_excl_scan_maps(F, s_wave, &agg);
+    u32 m = block_excl_scan_maps(F, s_wave, &agg);
+    u32 packed = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        packed |= map_pack(m) << (8 * e);
+        m = map_compose(f[e], m);
+    }
+    *(FMH_GLOBAL u32*)gmem(A.fmap + tile * CH + (u64)tid * E) = packed;
     if (tid == 0) A.chunk_map[tile] = agg;
 }
 
@@ -1161,40 +1167,28 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(CarryArgs A) {
 __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     constexpr int E = kCarryPerThread;
     constexpr int CH = kCarryChunk;
-    __shared__ u32 s_wave[kThreads / 64];
-    __shared__ u8 s_c[CH];
     const int tid = threadIdx.x;
     const u64 tile = blockIdx.x;
     const ProductDesc& D = product(A.items, A.one, A.chunk_item ? A.chunk_item[tile] : 0u);
     const u64 k0 = (tile - D.chunk0) * (u64)CH;
-    const u64 out_len = D.na + D.nb;
-    const u32 packed = *(const u32*)(A.fmap + tile * CH + (u64)tid * E);
-    u32 F = kMapId;
-#pragma unroll
-    for (int e = 0; e < E; e++) F = map_compose(map_unpack((packed >> (8 * e)) & 0xFFu), F);
-    u32 agg;
-    const u32 excl = block_excl_scan_maps(F, s_wave, &agg);
-    u32 c = map_apply(excl, A.chunk_cin[tile]);
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        s_c[tid * E + e] = (u8)c;
-        c = map_apply(map_unpack((packed >> (8 * e)) & 0xFFu), c);
-    }
-    __syncthreads();
     // the chunk starts inside the product: k0 < out_len
+    const u64 lim = D.na + D.nb - k0;
+    const FMH_GLOBAL u8* fm = gmem(A.fmap + tile * CH);
+    const u32 cin = A.chunk_cin[tile];
     FMH_GLOBAL u64* out = gmem(D.out + k0);
-    const u64 lim = out_len - k0;
+    u32 b[E];
     u64 w[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const u64 i = (u64)(e * kThreads + tid);
+        b[e] = fm[i];
         w[e] = out[i < lim ? i : lim - 1];
     }
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const int i = e * kThreads + tid;
-        const u32 ci = s_c[i];
-        if (ci != 0 && (u64)i < lim) out[i] = w[e] + ci;
+        const u64 i = (u64)(e * kThreads + tid);
+        const u32 c = map_apply(map_unpack(b[e]), cin);
+        if (c != 0 && i < lim) out[i] = w[e] + c;
     }
 }
 
