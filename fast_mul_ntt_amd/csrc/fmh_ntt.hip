@@ -1164,6 +1164,15 @@ __global__ __launch_bounds__(kScanThreads) void k_chunk_scan(CarryArgs A) {
     }
 }
 
+// FUSED: the workgroup composes the maps of the chunks before its own
+// instead of reading k_chunk_scan's result (products of at most
+// kFusedChunks chunks; each workgroup then reads at most that many maps).
+#ifndef FMH_FUSED_CHUNKS
+#define FMH_FUSED_CHUNKS 2048  // the CPU emulation builds with 4 to reach both paths with short products
+#endif
+constexpr u32 kFusedChunks = FMH_FUSED_CHUNKS;
+
+template <bool FUSED>
 __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     constexpr int E = kCarryPerThread;
     constexpr int CH = kCarryChunk;
@@ -1174,7 +1183,23 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     // the chunk starts inside the product: k0 < out_len
     const u64 lim = D.na + D.nb - k0;
     const FMH_GLOBAL u8* fm = gmem(A.fmap + tile * CH);
-    const u32 cin = A.chunk_cin[tile];
+    u32 cin;
+    if (FUSED) {
+        // maps of chunks chunk0 .. tile - 1, thread t composing a contiguous
+        // run of them in order
+        __shared__ u32 s_wave[kThreads / 64];
+        const u32 before = (u32)(tile - D.chunk0);
+        const u32 per = (before + kThreads - 1) / kThreads;
+        const u32 beg = min(before, (u32)tid * per), end = min(before, beg + per);
+        const FMH_GLOBAL u32* maps = gmem(A.chunk_map + D.chunk0);
+        u32 seg = kMapId;
+        for (u32 i = beg; i < end; i++) seg = map_compose(maps[i], seg);
+        u32 all;
+        (void)block_excl_scan_maps(seg, s_wave, &all);
+        cin = map_apply(all, 0);
+    } else {
+        cin = A.chunk_cin[tile];
+    }
     FMH_GLOBAL u64* out = gmem(D.out + k0);
     u32 b[E];
     u64 w[E];
@@ -1678,8 +1703,14 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     K.chunk_map = j.chunk_map;
     K.chunk_cin = j.chunk_cin;
     FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kThreads), 0, s, K);
-    FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.crt.size()), dim3(kScanThreads), 0, s, K);
-    FMH_LAUNCH("carry_apply", k_carry_apply, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    u32 most = 0;  // chunks of the job's longest product
+    for (const auto& d : j.crt) most = std::max(most, d.nchunks);
+    if (most <= kFusedChunks) {
+        FMH_LAUNCH("carry_apply", k_carry_apply<true>, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    } else {
+        FMH_LAUNCH("chunk_scan", k_chunk_scan, dim3((unsigned)j.crt.size()), dim3(kScanThreads), 0, s, K);
+        FMH_LAUNCH("carry_apply", k_carry_apply<false>, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    }
     return hipSuccess;
 }
 
