@@ -587,19 +587,25 @@ struct RowLine {
     u32 active;
 };
 
-// Row-tile LDS layout.  Word i of a tile lives at row_swz(i): the words of
-// each aligned 16-word block are permuted by bits 4-7 of i (low three bits
-// XOR bits 4-6, bit 3 XOR the parity of bits 4-7).  Every access pattern of
-// the row pass then touches 16 distinct 8-byte bank pairs per 16 lanes:
-//   * rounds with half-span hl >= 16 and the global <-> LDS copies read 16
-//     consecutive aligned words (a permutation of one block);
-//   * the fused round reads word 8k + q in lane k (bits 4-6 select the low
-//     bits, bit 3 comes from k);
-//   * rounds with hl = 8 read words u + 8q of two neighbouring groups whose
-//     bits 4-7 differ in exactly one bit, so their bit-3 parities differ.
+// Row-tile LDS layout.  Word i of a tile lives at row_swz(i) = i XOR bits
+// 4-7 of i: the words of each aligned 16-word block are permuted by the
+// block's index.  With 64 four-byte banks serving 32 lanes of 8-byte words,
+// the row pass's access patterns then cost at most two passes:
+//   * the global <-> LDS copies and rounds with half-span hl >= 16 read 16
+//     or 32 consecutive aligned words per group of lanes (one pass);
+//   * the fused round reads word 8k + q in lane k, and rounds with hl = 8
+//     read u + 8q: consecutive lanes land in different 16-word blocks, which
+//     the XOR spreads over the banks (two passes per 32 lanes).
+// A parity-based variant that is conflict-free for 16-lane groups cost
+// four more VALU operations per address and ran the 2^21-point row pass
+// 7% slower (116.5 against 108.9 us).
+//
+// row_swz is linear over XOR, and a group's word offsets q 2^lq occupy bits
+// that its base leaves zero, so row_swz(base + (q << lq)) =
+// row_swz(base) ^ row_swz(q << lq): one per-lane value per group and a
+// wave-uniform one per word.
 __device__ __forceinline__ int row_swz(int i) {
-    const int m = (i >> 4) & 15;
-    return i ^ ((m & 7) | ((__builtin_popcount(m) & 1) << 3));
+    return i ^ ((i >> 4) & 15);
 }
 
 // One round of R consecutive stages on one operand tile (word c * L + j),
@@ -619,8 +625,9 @@ __device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, i
     const int c = rest >> lb;
     const int base = (c << logL) + (blk << (lq + R)) + u;
     u64 x[Q];
+    const int sb = row_swz(base);
 #pragma unroll
-    for (int q = 0; q < Q; q++) x[q] = t[row_swz(base + (q << lq))];
+    for (int q = 0; q < Q; q++) x[q] = t[sb ^ row_swz(q << lq)];
 #pragma unroll
     for (int st = 0; st < R; st++) {
         const int hq = DIF ? (1 << (R - 1 - st)) : (1 << st);
@@ -637,7 +644,7 @@ __device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, i
         }
     }
 #pragma unroll
-    for (int q = 0; q < Q; q++) t[row_swz(base + (q << lq))] = x[q];
+    for (int q = 0; q < Q; q++) t[sb ^ row_swz(q << lq)] = x[q];
 }
 
 // A round over the tiles of nt operands (t1 = second operand or null).
@@ -679,11 +686,12 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
     const bool inv = A.mode == kRowMul || A.mode == kRowSqr || A.mode == kRowMulSpec || A.mode == kRowSqrSpec;
     for (int g = threadIdx.x; g < (n >> R); g += A.threads) {
         const int base = g << R;
+        const int sb = row_swz(base);  // word base + q is at sb ^ q
         u64 x[Q], y[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            x[q] = tA[row_swz(base + q)];
-            if (two) y[q] = tB[row_swz(base + q)];
+            x[q] = tA[sb ^ q];
+            if (two) y[q] = tB[sb ^ q];
         }
         if (fwd) {
 #pragma unroll
@@ -705,7 +713,7 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
         }
         if (!inv) {
 #pragma unroll
-            for (int q = 0; q < Q; q++) tA[row_swz(base + q)] = x[q];
+            for (int q = 0; q < Q; q++) tA[sb ^ q] = x[q];
             continue;
         }
         if (A.mode == kRowMulSpec) {
@@ -732,7 +740,7 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
             }
         }
 #pragma unroll
-        for (int q = 0; q < Q; q++) tA[row_swz(base + q)] = x[q];
+        for (int q = 0; q < Q; q++) tA[sb ^ q] = x[q];
     }
     __syncthreads();
 }
