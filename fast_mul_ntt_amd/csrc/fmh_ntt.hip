@@ -357,6 +357,14 @@ __device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) 
 // << logB) + b0 + c].  Consecutive threads take consecutive c, then
 // consecutive j: a wave reads whole tile rows.
 
+// Per group the twiddle rows are read as uniform row pointers W + (k hl <<
+// logB) plus one per-lane 32-bit byte offset ((u << logB) + b0 + c) 8, so
+// the loads use a scalar base and no 64-bit address arithmetic per lane
+// (indices stay below 2^28, byte offsets below 2^31).
+__device__ __forceinline__ u64 load_at(const u64* base, u32 byte_off) {
+    return *(const FMH_GLOBAL u64*)((const FMH_GLOBAL char*)gmem(base) + byte_off);
+}
+
 // Radix-2^2 DIF rounds over the stages hl = L/2 .. 1 (one radix-2 stage last
 // when logL is odd).
 template <int P>
@@ -368,17 +376,20 @@ __device__ __forceinline__ void tile_dif(u64* __restrict__ t, int logL, int logC
         const int lq = rem - 2;
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
+        const int st = hl << logC;
+        const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
+        const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
+        const u64* __restrict__ Wc = W + ((u64)hl << logB);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             const int c = beta & (C - 1);
             const int r = beta >> logC;
             const int u = r & (hl - 1);
-            const int j0 = ((r >> lq) << (lq + 2)) + u;
-            const u32 bb = b0 + (u32)c;
-            const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
-            const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
-            const u64 wc = W[((u32)(hl + u) << logB) + bb];
+            const int j0 = ((r - u) << 2) + u;  // (r / hl) 4 hl + u
+            const u32 off = (((u32)u << logB) + b0 + (u32)c) << 3;
+            const u64 wa = load_at(Wa, off);
+            const u64 wb = load_at(Wb, off);
+            const u64 wc = load_at(Wc, off);
             const int i0 = (j0 << logC) + c;
-            const int st = hl << logC;
             u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
             bf_dif<P>(x0, x2, wa);
             bf_dif<P>(x1, x3, wb);
@@ -394,9 +405,10 @@ __device__ __forceinline__ void tile_dif(u64* __restrict__ t, int logL, int logC
     }
     if (rem == 1) {
         const int nbf = 1 << (logL - 1 + logC);
+        const u64* __restrict__ W1 = W + ((u64)1 << logB);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             const int c = beta & (C - 1);
-            const u64 w = W[(1u << logB) + b0 + (u32)c];
+            const u64 w = load_at(W1, (b0 + (u32)c) << 3);
             const int i0 = ((2 * (beta >> logC)) << logC) + c;
             u64 x0 = t[i0], x1 = t[i0 + C];
             bf_dif<P>(x0, x1, w);
@@ -415,9 +427,10 @@ __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC
     const int C = 1 << logC;
     if (logL & 1) {
         const int nbf = 1 << (logL - 1 + logC);
+        const u64* __restrict__ W1 = W + ((u64)1 << logB);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             const int c = beta & (C - 1);
-            const u64 w = W[(1u << logB) + b0 + (u32)c];
+            const u64 w = load_at(W1, (b0 + (u32)c) << 3);
             const int i0 = ((2 * (beta >> logC)) << logC) + c;
             u64 x0 = t[i0], x1 = t[i0 + C];
             bf_dit<P>(x0, x1, w);
@@ -429,17 +442,20 @@ __device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC
     for (int lq = logL & 1; lq + 2 <= logL; lq += 2) {
         const int hl = 1 << lq;
         const int nbf = 1 << (logL - 2 + logC);
+        const int st = hl << logC;
+        const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
+        const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
+        const u64* __restrict__ Wc = W + ((u64)hl << logB);
         for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
             const int c = beta & (C - 1);
             const int r = beta >> logC;
             const int u = r & (hl - 1);
-            const int j0 = ((r >> lq) << (lq + 2)) + u;
-            const u32 bb = b0 + (u32)c;
-            const u64 wa = W[((u32)(2 * hl + u) << logB) + bb];
-            const u64 wb = W[((u32)(3 * hl + u) << logB) + bb];
-            const u64 wc = W[((u32)(hl + u) << logB) + bb];
+            const int j0 = ((r - u) << 2) + u;  // (r / hl) 4 hl + u
+            const u32 off = (((u32)u << logB) + b0 + (u32)c) << 3;
+            const u64 wa = load_at(Wa, off);
+            const u64 wb = load_at(Wb, off);
+            const u64 wc = load_at(Wc, off);
             const int i0 = (j0 << logC) + c;
-            const int st = hl << logC;
             u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
             bf_dit<P>(x0, x1, wc);
             bf_dit<P>(x2, x3, wc);
