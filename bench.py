@@ -40,20 +40,20 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against the copy
 # kernel) from rocprofv3 --pmc runs of this workload; profiles/ holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 205407093.0,
-    "row_mul": 154679417.0,
-    "col_inv": 151170110.0,
-    "crt_local": 70275494.0,
-    "carry_apply": 34716665.0,
+    "col_fwd": 205333400.0,
+    "row_mul": 154585187.0,
+    "col_inv": 151204459.0,
+    "crt_local": 70275187.0,
+    "carry_apply": 34716692.0,
 }
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
 # the two column passes).
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3559799419.0,
-    "row_mul": 2418619213.0,
-    "col_inv": 2200933828.0,
-    "crt_local": 1121026137.0,
-    "carry_apply": 555747963.0,
+    "col_fwd": 3560709083.0,
+    "row_mul": 2418188376.0,
+    "col_inv": 2202057931.0,
+    "crt_local": 1121036584.0,
+    "carry_apply": 555751545.0,
 }
 
 
