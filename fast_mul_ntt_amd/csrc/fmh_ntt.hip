@@ -365,108 +365,112 @@ __device__ __forceinline__ u64 load_at(const u64* base, u32 byte_off) {
     return *(const FMH_GLOBAL u64*)((const FMH_GLOBAL char*)gmem(base) + byte_off);
 }
 
-// Radix-2^2 DIF rounds over the stages hl = L/2 .. 1 (one radix-2 stage last
-// when logL is odd).
-template <int P>
-__device__ __forceinline__ void tile_dif(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
-                                         const u64* __restrict__ W, int nthr) {
+// One round of a column tile: R = 2 (radix-2^2, half-spans 2 hl and hl) or
+// R = 1 (radix-2, half-span hl) over every group of 2^R words (c, j0 + q hl),
+// q < 2^R, of line c.  ld(e) reads tile word e = j C + c, fix(e, v) gives its
+// value and st(e, v) writes it back: to and from the tile in LDS, or the
+// transform's words in global memory for the first and the last round of a
+// tile.  G groups of a thread issue their loads together (an index past the
+// end is clamped and its result dropped).
+template <int P, int R, bool DIF, int G, class Ld, class Fix, class St>
+__device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, const u64* __restrict__ W, int lq,
+                                          int nthr, Ld ld, Fix fix, St st) {
+    constexpr int K = 1 << R;
     const int C = 1 << logC;
-    int rem = logL;
-    while (rem >= 2) {
-        const int lq = rem - 2;
-        const int hl = 1 << lq;
-        const int nbf = 1 << (logL - 2 + logC);
-        const int st = hl << logC;
-        const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
-        const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
-        const u64* __restrict__ Wc = W + ((u64)hl << logB);
-        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
+    const int hl = 1 << lq;
+    const int nbf = 1 << (logL - R + logC);
+    const int sp = hl << logC;  // tile distance of words q and q + 1 of a group
+    const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
+    const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
+    const u64* __restrict__ Wc = W + ((u64)hl << logB);
+    for (int beta0 = threadIdx.x; beta0 < nbf; beta0 += G * nthr) {
+        u64 x[G][K], w[G][R == 2 ? 3 : 1];
+        int i0[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int beta = G > 1 ? min(beta0 + k * nthr, nbf - 1) : beta0;
             const int c = beta & (C - 1);
             const int r = beta >> logC;
             const int u = r & (hl - 1);
-            const int j0 = ((r - u) << 2) + u;  // (r / hl) 4 hl + u
+            const int j0 = ((r - u) << R) + u;  // (r / hl) 2^R hl + u
             const u32 off = (((u32)u << logB) + b0 + (u32)c) << 3;
-            const u64 wa = load_at(Wa, off);
-            const u64 wb = load_at(Wb, off);
-            const u64 wc = load_at(Wc, off);
-            const int i0 = (j0 << logC) + c;
-            u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
-            bf_dif<P>(x0, x2, wa);
-            bf_dif<P>(x1, x3, wb);
-            bf_dif<P>(x0, x1, wc);
-            bf_dif<P>(x2, x3, wc);
-            t[i0] = x0;
-            t[i0 + st] = x1;
-            t[i0 + 2 * st] = x2;
-            t[i0 + 3 * st] = x3;
+            w[k][0] = load_at(Wc, off);
+            if (R == 2) {
+                w[k][R == 2 ? 1 : 0] = load_at(Wa, off);
+                w[k][R == 2 ? 2 : 0] = load_at(Wb, off);
+            }
+            i0[k] = (j0 << logC) + c;
+#pragma unroll
+            for (int q = 0; q < K; q++) x[k][q] = ld(i0[k] + q * sp);
         }
-        __syncthreads();
-        rem -= 2;
-    }
-    if (rem == 1) {
-        const int nbf = 1 << (logL - 1 + logC);
-        const u64* __restrict__ W1 = W + ((u64)1 << logB);
-        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            const int c = beta & (C - 1);
-            const u64 w = load_at(W1, (b0 + (u32)c) << 3);
-            const int i0 = ((2 * (beta >> logC)) << logC) + c;
-            u64 x0 = t[i0], x1 = t[i0 + C];
-            bf_dif<P>(x0, x1, w);
-            t[i0] = x0;
-            t[i0 + C] = x1;
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            if (G > 1 && beta0 + k * nthr >= nbf) break;
+#pragma unroll
+            for (int q = 0; q < K; q++) x[k][q] = fix(i0[k] + q * sp, x[k][q]);
+            u64* y = x[k];
+            if (R == 1) {
+                if (DIF)
+                    bf_dif<P>(y[0], y[1], w[k][0]);
+                else
+                    bf_dit<P>(y[0], y[1], w[k][0]);
+            } else if (DIF) {
+                bf_dif<P>(y[0], y[K > 2 ? 2 : 0], w[k][R == 2 ? 1 : 0]);
+                bf_dif<P>(y[1], y[K > 2 ? 3 : 1], w[k][R == 2 ? 2 : 0]);
+                bf_dif<P>(y[0], y[1], w[k][0]);
+                bf_dif<P>(y[K > 2 ? 2 : 0], y[K > 2 ? 3 : 1], w[k][0]);
+            } else {
+                bf_dit<P>(y[0], y[1], w[k][0]);
+                bf_dit<P>(y[K > 2 ? 2 : 0], y[K > 2 ? 3 : 1], w[k][0]);
+                bf_dit<P>(y[0], y[K > 2 ? 2 : 0], w[k][R == 2 ? 1 : 0]);
+                bf_dit<P>(y[1], y[K > 2 ? 3 : 1], w[k][R == 2 ? 2 : 0]);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) st(i0[k] + q * sp, x[k][q]);
         }
-        __syncthreads();
     }
 }
 
-// The inverse: one radix-2 stage first when logL is odd, then radix-2^2 DIT
-// rounds up to hl = L/2.
-template <int P>
-__device__ __forceinline__ void tile_dit(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
-                                         const u64* __restrict__ W, int nthr) {
-    const int C = 1 << logC;
-    if (logL & 1) {
-        const int nbf = 1 << (logL - 1 + logC);
-        const u64* __restrict__ W1 = W + ((u64)1 << logB);
-        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            const int c = beta & (C - 1);
-            const u64 w = load_at(W1, (b0 + (u32)c) << 3);
-            const int i0 = ((2 * (beta >> logC)) << logC) + c;
-            u64 x0 = t[i0], x1 = t[i0 + C];
-            bf_dit<P>(x0, x1, w);
-            t[i0] = x0;
-            t[i0 + C] = x1;
+// Round `first` of a tile reads the transform (ld, fix), round `last` writes
+// it (st); the rounds between go through the tile t in LDS.
+template <int P, int R, bool DIF, class Ld, class Fix, class St>
+__device__ __forceinline__ void col_round_io(bool first, bool last, u64* __restrict__ t, int logL, int logC, int logB,
+                                             u32 b0, const u64* __restrict__ W, int lq, int nthr, Ld ld, Fix fix,
+                                             St st) {
+    // four data words of the first round in flight per thread: eight (two
+    // radix-4 groups) took the kernel from 42 to 94 VGPRs and ran slower
+    constexpr int GI = 4 >> R;
+    auto tld = [&](int e) { return t[e]; };
+    auto same = [](int, u64 v) { return v; };
+    auto tst = [&](int e, u64 v) { t[e] = v; };
+    if (first && last)
+        col_round<P, R, DIF, GI>(logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
+    else if (first)
+        col_round<P, R, DIF, GI>(logL, logC, logB, b0, W, lq, nthr, ld, fix, tst);
+    else if (last)
+        col_round<P, R, DIF, 1>(logL, logC, logB, b0, W, lq, nthr, tld, same, st);
+    else
+        col_round<P, R, DIF, 1>(logL, logC, logB, b0, W, lq, nthr, tld, same, tst);
+    if (!last) __syncthreads();
+}
+
+// The stages of a column tile (logL >= 1): DIF as radix-2^2 rounds over
+// hl = L/2 .. 1 with one radix-2 stage last when logL is odd; DIT as the
+// radix-2 stage first when logL is odd, then radix-2^2 rounds up to hl = L/2.
+template <int P, bool DIF, class Ld, class Fix, class St>
+__device__ __forceinline__ void tile_rounds(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
+                                            const u64* __restrict__ W, int nthr, Ld ld, Fix fix, St st) {
+    const int nr = (logL >> 1) + (logL & 1);
+    const bool odd = logL & 1;
+    for (int i = 0; i < nr; i++) {
+        const bool first = i == 0, last = i == nr - 1;
+        const bool r1 = odd && (DIF ? last : first);
+        if (r1) {
+            col_round_io<P, 1, DIF>(first, last, t, logL, logC, logB, b0, W, 0, nthr, ld, fix, st);
+        } else {
+            const int lq = DIF ? logL - 2 * (i + 1) : (logL & 1) + 2 * (i - (odd ? 1 : 0));
+            col_round_io<P, 2, DIF>(first, last, t, logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
         }
-        __syncthreads();
-    }
-    for (int lq = logL & 1; lq + 2 <= logL; lq += 2) {
-        const int hl = 1 << lq;
-        const int nbf = 1 << (logL - 2 + logC);
-        const int st = hl << logC;
-        const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
-        const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
-        const u64* __restrict__ Wc = W + ((u64)hl << logB);
-        for (int beta = threadIdx.x; beta < nbf; beta += nthr) {
-            const int c = beta & (C - 1);
-            const int r = beta >> logC;
-            const int u = r & (hl - 1);
-            const int j0 = ((r - u) << 2) + u;  // (r / hl) 4 hl + u
-            const u32 off = (((u32)u << logB) + b0 + (u32)c) << 3;
-            const u64 wa = load_at(Wa, off);
-            const u64 wb = load_at(Wb, off);
-            const u64 wc = load_at(Wc, off);
-            const int i0 = (j0 << logC) + c;
-            u64 x0 = t[i0], x1 = t[i0 + st], x2 = t[i0 + 2 * st], x3 = t[i0 + 3 * st];
-            bf_dit<P>(x0, x1, wc);
-            bf_dit<P>(x2, x3, wc);
-            bf_dit<P>(x0, x2, wa);
-            bf_dit<P>(x1, x3, wb);
-            t[i0] = x0;
-            t[i0 + st] = x1;
-            t[i0 + 2 * st] = x2;
-            t[i0 + 3 * st] = x3;
-        }
-        __syncthreads();
     }
 }
 
@@ -528,25 +532,22 @@ __device__ void col_body(const ColArgs& A) {
     const u64 len = op ? D.nb : D.na;
     // tile word e = j C + c is word base + j B + c of the transform
     auto at = [&](int e) { return base + ((u64)(e >> logC) << logB) + (u64)(e & (C - 1)); };
+    auto put = [&](int e, u64 v) { res[at(e)] = v; };
     if (!A.from_limbs) {
-        batched_copy(n, nthr, [&](int e) { return res[at(e)]; }, [&](int e, u64 v) { lds[e] = v; });
-    } else if (len > 0) {
-        batched_copy(
-            n, nthr,
-            [&](int e) {
-                const u64 g = at(e);
-                return limbs[g < len ? g : len - 1];
-            },
-            [&](int e, u64 v) { lds[e] = at(e) < len ? reduce_word<P>(v) : 0ull; });
+        auto get = [&](int e) { return res[at(e)]; };
+        auto same = [](int, u64 v) { return v; };
+        tile_rounds<P, FWD>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, same, put);
     } else {
-        for (int e = threadIdx.x; e < n; e += nthr) lds[e] = 0;
+        // an empty operand reads the first twiddle and drops it
+        const FMH_GLOBAL u64* __restrict__ src = len ? limbs : gmem(A.tw[P]);
+        const u64 last = len ? len - 1 : 0;
+        auto get = [&](int e) {
+            const u64 g = at(e);
+            return src[g < last ? g : last];
+        };
+        auto val = [&](int e, u64 v) { return at(e) < len ? reduce_word<P>(v) : 0ull; };
+        tile_rounds<P, FWD>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, val, put);
     }
-    __syncthreads();
-    if (FWD)
-        tile_dif<P>(lds, logL, logC, logB, b0, A.tw[P], nthr);
-    else
-        tile_dit<P>(lds, logL, logC, logB, b0, A.tw[P], nthr);
-    batched_copy(n, nthr, [&](int e) { return lds[e]; }, [&](int e, u64 v) { res[at(e)] = v; });
 }
 
 template <bool FWD>
@@ -629,21 +630,21 @@ __device__ __forceinline__ int row_swz(int i) {
 // DIT (increasing), hl = 2^lq.  Group g holds the 2^R words
 // base + q hl of one line; the twiddle of the pair (q, q + hq) at stage
 // half-span H = hq hl is W[H + (j mod H)] = W[(hq + q mod hq) hl + u].
-template <int P, int R, bool DIF>
-__device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, int lq, int g,
-                                                const u64* __restrict__ W) {
-    constexpr int Q = 1 << R;
-    const int hl = 1 << lq;
-    const int u = g & (hl - 1);
+// Tile word of the first element of group g (of one operand).
+template <int R>
+__device__ __forceinline__ int row_group_base(int logL, int lq, int g) {
+    const int u = g & ((1 << lq) - 1);
     const int rest = g >> lq;
     const int lb = logL - lq - R;  // log2 of groups per line
     const int blk = rest & ((1 << lb) - 1);
     const int c = rest >> lb;
-    const int base = (c << logL) + (blk << (lq + R)) + u;
-    u64 x[Q];
-    const int sb = row_swz(base);
-#pragma unroll
-    for (int q = 0; q < Q; q++) x[q] = t[sb ^ row_swz(q << lq)];
+    return (c << logL) + (blk << (lq + R)) + u;
+}
+
+// The R stages of a group in registers; u = base mod hl.
+template <int P, int R, bool DIF>
+__device__ __forceinline__ void row_group_stages(u64 (&x)[1 << R], int lq, int u, const u64* __restrict__ W) {
+    constexpr int Q = 1 << R;
 #pragma unroll
     for (int st = 0; st < R; st++) {
         const int hq = DIF ? (1 << (R - 1 - st)) : (1 << st);
@@ -659,6 +660,18 @@ __device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, i
                 bf_dit<P>(x[q], x[q + hq], w[q & (hq - 1)]);
         }
     }
+}
+
+template <int P, int R, bool DIF>
+__device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, int lq, int g,
+                                                const u64* __restrict__ W) {
+    constexpr int Q = 1 << R;
+    const int base = row_group_base<R>(logL, lq, g);
+    u64 x[Q];
+    const int sb = row_swz(base);
+#pragma unroll
+    for (int q = 0; q < Q; q++) x[q] = t[sb ^ row_swz(q << lq)];
+    row_group_stages<P, R, DIF>(x, lq, base & ((1 << lq) - 1), W);
 #pragma unroll
     for (int q = 0; q < Q; q++) t[sb ^ row_swz(q << lq)] = x[q];
 }
@@ -683,6 +696,104 @@ __device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, in
         case 2: row_round<P, 2, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
         case 3: row_round<P, 3, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
         default: row_round<P, (RMAX > 3 ? 4 : 3), DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
+    }
+}
+
+// The first forward round of a row tile, read straight from the operands'
+// sources instead of a filled tile: group g (of operand g / per) loads its
+// 2^R words of line c, runs the R DIF stages and writes them to the tile.
+// The G groups of one step issue all their loads first (about kBatch words
+// in flight per thread); a group index past the end is clamped and its
+// result dropped.  Word j of line c of operand op is
+// src[op][min(g0 + j, last[op])], zero when the line is inactive or j is
+// past len[op], reduced mod p when the sources are limbs.
+template <int P, int R>
+__device__ __forceinline__ void row_round_in(const RowLine* s_line, bool reduce, u64* t0, u64* t1, int nt, int n,
+                                             int logL, int lq, const u64* __restrict__ W, int nthr) {
+    constexpr int Q = 1 << R;
+    constexpr int G = Q >= kBatch ? 1 : kBatch / Q;
+    const int per = n >> R;
+    const int total = nt * per;
+    const int L = 1 << logL;
+    for (int g0 = threadIdx.x; g0 < total; g0 += G * nthr) {
+        u64 x[G][Q];
+        int base[G], op[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int g = min(g0 + k * nthr, total - 1);
+            op[k] = g >= per ? 1 : 0;
+            base[k] = row_group_base<R>(logL, lq, g - op[k] * per);
+            const RowLine& li = s_line[base[k] >> logL];
+            const u64 j0 = li.g0 + (u64)(base[k] & (L - 1));
+            const u64 last = li.last[op[k]];
+            const FMH_GLOBAL u64* src = gmem(li.src[op[k]]);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const u64 j = j0 + ((u64)q << lq);
+                x[k][q] = src[j < last ? j : last];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            if (g0 + k * nthr >= total) break;
+            const RowLine& li = s_line[base[k] >> logL];
+            const u64 j0 = li.g0 + (u64)(base[k] & (L - 1));
+            const u64 len = li.active ? li.len[op[k]] : 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const u64 j = j0 + ((u64)q << lq);
+                x[k][q] = j >= len ? 0ull : reduce ? reduce_word<P>(x[k][q]) : x[k][q];
+            }
+            row_group_stages<P, R, true>(x[k], lq, base[k] & ((1 << lq) - 1), W);
+            u64* __restrict__ t = op[k] ? t1 : t0;
+            const int sb = row_swz(base[k]);
+#pragma unroll
+            for (int q = 0; q < Q; q++) t[sb ^ row_swz(q << lq)] = x[k][q];
+        }
+    }
+    __syncthreads();
+}
+
+// The last inverse round of a row tile, written straight to the residues of
+// each active line instead of back to the tile.
+template <int P, int R>
+__device__ __forceinline__ void row_round_out(const RowLine* s_line, const u64* __restrict__ t, int n, int logL,
+                                              int lq, const u64* __restrict__ W, int nthr) {
+    constexpr int Q = 1 << R;
+    const int L = 1 << logL;
+    for (int g = threadIdx.x; g < (n >> R); g += nthr) {
+        const int base = row_group_base<R>(logL, lq, g);
+        u64 x[Q];
+        const int sb = row_swz(base);
+#pragma unroll
+        for (int q = 0; q < Q; q++) x[q] = t[sb ^ row_swz(q << lq)];
+        row_group_stages<P, R, false>(x, lq, base & ((1 << lq) - 1), W);
+        const RowLine& li = s_line[base >> logL];
+        if (!li.active) continue;
+        FMH_GLOBAL u64* out = gmem(li.ra + li.g0 + (u64)(base & (L - 1)));
+#pragma unroll
+        for (int q = 0; q < Q; q++) out[q << lq] = x[q];
+    }
+}
+
+template <int P, int RMAX>
+__device__ void row_round_in_r(int r, const RowLine* s_line, bool reduce, u64* t0, u64* t1, int nt, int n,
+                               int logL, int lq, const u64* W, int nthr) {
+    switch (r) {
+        case 1: row_round_in<P, 1>(s_line, reduce, t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 2: row_round_in<P, 2>(s_line, reduce, t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 3: row_round_in<P, 3>(s_line, reduce, t0, t1, nt, n, logL, lq, W, nthr); break;
+        default: row_round_in<P, (RMAX > 3 ? 4 : 3)>(s_line, reduce, t0, t1, nt, n, logL, lq, W, nthr); break;
+    }
+}
+
+template <int P>
+__device__ void row_round_out_r(int r, const RowLine* s_line, const u64* t, int n, int logL, int lq,
+                                const u64* W, int nthr) {
+    switch (r) {
+        case 1: row_round_out<P, 1>(s_line, t, n, logL, lq, W, nthr); break;
+        case 2: row_round_out<P, 2>(s_line, t, n, logL, lq, W, nthr); break;
+        default: row_round_out<P, 3>(s_line, t, n, logL, lq, W, nthr); break;
     }
 }
 
@@ -846,21 +957,30 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
                 });
         }
     };
-    fill(tA, 0);
-    if (two) fill(tB, 1);
-    __syncthreads();
     // Stage schedule: the last RF forward stages, the pointwise product and
     // the first RF inverse stages run fused in registers; the other stages
     // run as rounds of up to 4 stages (both operands) or 3 (one operand).
+    // The first forward round reads the sources itself and the last inverse
+    // round writes the residues itself; a tile with no such round is filled
+    // and stored by plain copies.
     const int RF = logL < 3 ? logL : 3;
     const int nt = two ? 2 : 1;
+    const bool fold_in = !spec_in && logL > RF;
+    if (!fold_in) {
+        fill(tA, 0);
+        if (two) fill(tB, 1);
+        __syncthreads();
+    }
     if (!spec_in) {
         const int rmax = two ? RMAX : 3;
         int top = logL;
         while (top > RF) {
             const int left = top - RF;
             const int r = left % rmax ? left % rmax : rmax;
-            row_round_r<P, true, RMAX>(r, tA, tB, nt, n, logL, top - r, A.tw[P], nthr);
+            if (top == logL)
+                row_round_in_r<P, RMAX>(r, s_line, A.from_limbs != 0, tA, tB, nt, n, logL, top - r, A.tw[P], nthr);
+            else
+                row_round_r<P, true, RMAX>(r, tA, tB, nt, n, logL, top - r, A.tw[P], nthr);
             top -= r;
         }
     }
@@ -884,9 +1004,13 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
     }
     for (int lq = RF; lq < logL;) {
         const int r = logL - lq < 3 ? logL - lq : 3;
-        row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P], nthr);
+        if (lq + r == logL)
+            row_round_out_r<P>(r, s_line, tA, n, logL, lq, A.itw[P], nthr);
+        else
+            row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P], nthr);
         lq += r;
     }
+    if (logL > RF) return;
     if (by_line) {
         for (int c = 0; c < C; c++) {
             const RowLine li = s_line[c];
