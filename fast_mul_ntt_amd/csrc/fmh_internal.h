@@ -16,11 +16,15 @@ namespace fmh {
 // 2^27 + 2^27 limbs).  The reference's own limit is 2^55 (src/data.h:69).
 constexpr int kMaxLogN = 28;
 
-constexpr int kThreads = 256;         // threads per workgroup, CRT and carry kernels
+constexpr int kThreads = 256;         // threads per workgroup of the carry kernels
 constexpr int kScanThreads = 512;     // threads of the chunk scan (one workgroup per product)
 constexpr int kMaxPassThreads = 1024; // launch bound of the NTT pass kernels
-constexpr int kCarryPerThread = 4;    // limbs per thread in the carry scan
-constexpr int kCarryChunk = kThreads * kCarryPerThread;
+constexpr int kCarryPerThread = 4;    // limbs per thread in the carry kernels
+constexpr int kCarryChunk = kThreads * kCarryPerThread;  // limbs per chunk
+// The CRT kernel covers a chunk with more threads and fewer coefficients
+// each (fewer registers, more waves): 2^21-point products 25.2 -> 22.5 us.
+constexpr int kCrtThreads = 512;
+constexpr int kCrtPerThread = kCarryChunk / kCrtThreads;
 
 // Tile geometry of the NTT passes, per transform length 2^log_n.  Sources,
 // later ones overriding earlier ones:

@@ -1167,27 +1167,36 @@ __device__ __forceinline__ u32 block_excl_scan_maps(u32 F, u32* s_wave, u32* agg
     return map_compose(excl, wpre);
 }
 
-// Coefficients are computed with thread-strided (coalesced) indices into LDS;
-// the limb scan then reads LDS with kCarryPerThread consecutive limbs per
-// thread, and results go back to HBM through LDS with coalesced stores.
-static_assert(kCarryPerThread == 4, "limb maps of one thread are packed in one u32");
+// Chunk layout: limb i = e kCrtThreads + t of a chunk (e < kCrtPerThread) is
+// handled by thread t, so every load and store of a row e is coalesced.  Limb
+// i needs words 1 and 2 of coefficients i - 1 and i - 2: lanes below in the
+// same wave (shuffles), or, for lanes 0 and 1, the last two lanes of the
+// previous 64-limb segment (through LDS; the chunk's first segment takes
+// coefficients k0 - 2 and k0 - 1, computed by threads 0 and 1).  The carry
+// maps are scanned per segment in the wave, then across the segments in
+// limb order.
+__device__ __forceinline__ u64 shfl_up_u64(u64 v, unsigned d) {
+    const u32 lo = __shfl_up((unsigned)v, d, 64), hi = __shfl_up((unsigned)(v >> 32), d, 64);
+    return ((u64)hi << 32) | lo;
+}
 
-__global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
-    constexpr int E = kCarryPerThread;
+__global__ __launch_bounds__(kCrtThreads) void k_crt_local(CarryArgs A) {
+    constexpr int E = kCrtPerThread;
     constexpr int CH = kCarryChunk;
-    __shared__ u64 s_w0[CH];
-    __shared__ u64 s_w1[CH + 2];
-    __shared__ u64 s_w2[CH + 2];
-    __shared__ u32 s_wave[kThreads / 64];
-    const int tid = threadIdx.x;
+    constexpr int NW = kCrtThreads / 64;
+    constexpr int NS = E * NW;  // 64-limb segments per chunk, segment e NW + wave
+    // s_edge[s + 1]: words 1 and 2 of the coefficients of lanes 62 and 63 of
+    // segment s; s_edge[0]: those of coefficients k0 - 2 and k0 - 1
+    __shared__ u64 s_edge[NS + 1][4];
+    __shared__ u32 s_agg[NS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u64 tile = blockIdx.x;
     const ProductDesc& D = product(A.items, A.one, A.chunk_item ? A.chunk_item[tile] : 0u);
     const u64 k0 = (tile - D.chunk0) * (u64)CH;
     const u64 out_len = D.na + D.nb;
 
     u64 v[E][3];
-    // threads 0 and 1 also take coefficient k0 - 2 + tid, whose high words
-    // reach into the chunk's first limbs
+    // threads 0 and 1 also take coefficient k0 - 2 + tid
     const bool before = tid < 2 && k0 >= (u64)(2 - tid);
     u64 x[3] = {0, 0, 0};
     if (!D.split_n) {
@@ -1198,7 +1207,7 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         const FMH_GLOBAL u64* res = gmem(D.res);
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const u64 k = k0 + (u64)(e * kThreads + tid);
+            const u64 k = k0 + (u64)(e * kCrtThreads + tid);
             const u64 kc = k < kend ? k : kend - 1;
             v[e][0] = res[kc];
             v[e][1] = res[kc + N];
@@ -1210,67 +1219,77 @@ __global__ __launch_bounds__(kThreads) void k_crt_local(CarryArgs A) {
         x[2] = res[kx + 2 * N];
 #pragma unroll
         for (int e = 0; e < E; e++)
-            if (k0 + (u64)(e * kThreads + tid) >= kend) v[e][0] = v[e][1] = v[e][2] = 0;
+            if (k0 + (u64)(e * kCrtThreads + tid) >= kend) v[e][0] = v[e][1] = v[e][2] = 0;
         if (!before) x[0] = x[1] = x[2] = 0;
     } else {
         if (before) crt_load(D, k0 - 2 + tid, x[0], x[1], x[2]);
 #pragma unroll
-        for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kThreads + tid), v[e][0], v[e][1], v[e][2]);
+        for (int e = 0; e < E; e++) crt_load(D, k0 + (u64)(e * kCrtThreads + tid), v[e][0], v[e][1], v[e][2]);
     }
+    u64 w0[E], w1[E], w2[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const int i = e * kThreads + tid;
-        u64 w0, w1, w2;
-        crt_coef(D.crt, v[e][0], v[e][1], v[e][2], w0, w1, w2);
-        s_w0[i] = w0;
-        s_w1[i + 2] = w1;
-        s_w2[i + 2] = w2;
+        crt_coef(D.crt, v[e][0], v[e][1], v[e][2], w0[e], w1[e], w2[e]);
+        if (lane >= 62) {
+            s_edge[e * NW + wave + 1][2 * (lane - 62)] = w1[e];
+            s_edge[e * NW + wave + 1][2 * (lane - 62) + 1] = w2[e];
+        }
     }
     if (tid < 2) {
-        u64 w0, w1, w2;
-        crt_coef(D.crt, x[0], x[1], x[2], w0, w1, w2);
-        s_w1[tid] = w1;
-        s_w2[tid] = w2;
+        u64 c0, c1, c2;
+        crt_coef(D.crt, x[0], x[1], x[2], c0, c1, c2);
+        s_edge[0][2 * tid] = c1;
+        s_edge[0][2 * tid + 1] = c2;
     }
     __syncthreads();
 
-    u32 F = kMapId;
-    u32 f[E];
-    u64 lo[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int i = tid * E + e;
-        const u128 s = (u128)s_w0[i] + s_w1[i + 1] + s_w2[i];
-        const u64 l = (u64)s;
-        const u32 h = (u32)(s >> 64);
-        // byte c: h + [l + c >= 2^64]
-        f[e] = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
-               ((l >= ~0ull - 2 ? 1u : 0u) << 24);
-        F = map_compose(f[e], F);
-        lo[e] = l;
-    }
-    __syncthreads();  // all reads of s_w0 done
-#pragma unroll
-    for (int e = 0; e < E; e++) s_w0[tid * E + e] = lo[e];
-    __syncthreads();
+    u32 f[E], incl[E];
     FMH_GLOBAL u64* out = gmem(D.out);
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const int i = e * kThreads + tid;
-        if (k0 + (u64)i < out_len) out[k0 + i] = s_w0[i];
+        const int sg = e * NW + wave;
+        u64 u1 = shfl_up_u64(w1[e], 1), u2 = shfl_up_u64(w2[e], 2);
+        if (lane == 0) {
+            u1 = s_edge[sg][2];
+            u2 = s_edge[sg][1];
+        } else if (lane == 1) {
+            u2 = s_edge[sg][3];
+        }
+        const u128 sum = (u128)w0[e] + u1 + u2;
+        const u64 l = (u64)sum;
+        const u32 h = (u32)(sum >> 64);
+        // byte c: h + [l + c >= 2^64]
+        f[e] = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
+               ((l >= ~0ull - 2 ? 1u : 0u) << 24);
+        const u64 k = k0 + (u64)(e * kCrtThreads + tid);
+        if (k < out_len) out[k] = l;
+        // inclusive scan of the segment's maps in lane order
+        u32 m = f[e];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const u32 o = __shfl_up(m, d, 64);
+            if (lane >= d) m = map_compose(m, o);
+        }
+        incl[e] = m;
+        if (lane == 63) s_agg[sg] = m;
     }
-
-    // carry-in of limb tid E + e as a map of the chunk's carry-in
-    u32 agg;
-    u32 m = block_excl_scan_maps(F, s_wave, &agg);
-    u32 packed = 0;
+    __syncthreads();
+    // maps of the segments before each of this thread's segments
+    u32 run = kMapId, pre[E];
+#pragma unroll
+    for (int sg = 0; sg < NS; sg++) {
+        if (sg % NW == wave) pre[sg / NW] = run;
+        run = map_compose(s_agg[sg], run);
+    }
+    // carry-in of each limb as a map of the chunk's carry-in
+    FMH_GLOBAL u8* fm = gmem(A.fmap + tile * CH);
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        packed |= map_pack(m) << (8 * e);
-        m = map_compose(f[e], m);
+        u32 ex = __shfl_up(incl[e], 1, 64);
+        if (lane == 0) ex = kMapId;
+        fm[e * kCrtThreads + tid] = (u8)map_pack(map_compose(ex, pre[e]));
     }
-    *(FMH_GLOBAL u32*)gmem(A.fmap + tile * CH + (u64)tid * E) = packed;
-    if (tid == 0) A.chunk_map[tile] = agg;
+    if (tid == 0) A.chunk_map[tile] = run;
 }
 
 // One workgroup per product: the carry into each of its chunks.  The chunk
@@ -1850,7 +1869,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     K.fmap = j.fmap;
     K.chunk_map = j.chunk_map;
     K.chunk_cin = j.chunk_cin;
-    FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kThreads), 0, s, K);
+    FMH_LAUNCH("crt_local", k_crt_local, dim3(j.chunks), dim3(kCrtThreads), 0, s, K);
     u32 most = 0;  // chunks of the job's longest product
     for (const auto& d : j.crt) most = std::max(most, d.nchunks);
     if (most <= kFusedChunks) {
