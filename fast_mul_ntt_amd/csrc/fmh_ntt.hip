@@ -1054,12 +1054,14 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
 // map is packed into one byte as four 2-bit values.
 //
 // Three kernels, no inter-workgroup waiting and no contended atomics:
-//   k_crt_local   one chunk of limbs per workgroup: CRT, s_k, writes
-//                 lo(s_k) into out[], for each limb the map from the chunk's
-//                 carry-in to the limb's carry-in into fmap[], and the
-//                 chunk's composed map into chunk_map[]
+//   k_crt_local   one chunk of limbs per workgroup: CRT, s_k, for each limb
+//                 the map from the chunk's carry-in to the limb's carry-in
+//                 into fmap[], and the chunk's composed map into
+//                 chunk_map[]; writes lo(s_k) into out[], plus the carry-in
+//                 where that map is constant (the limb is then final)
 //   k_chunk_scan  one workgroup: the carry into every chunk, chunk_cin[]
-//   k_carry_apply adds fmap[k](chunk_cin) into each limb of out[].
+//   k_carry_apply adds fmap[k](chunk_cin) into each limb of out[] whose map
+//                 is not constant (in practice the chunk's first limb).
 
 struct CarryArgs {
     const ProductDesc* items;
@@ -1088,6 +1090,8 @@ __device__ __forceinline__ u32 map_pack(u32 m) {
 __device__ __forceinline__ u32 map_unpack(u32 b) {
     return (b & 3u) | ((b & 0xCu) << 6) | ((b & 0x30u) << 12) | ((b & 0xC0u) << 18);
 }
+// A packed map that sends every carry to the same value.
+__device__ __forceinline__ bool map_const(u32 b) { return b == (b & 3u) * 0x55u; }
 
 // Residues of coefficient k (zero beyond the convolution).  For a split
 // product, coefficient k sums those of piece i = k / m at offset k - i m and
@@ -1244,7 +1248,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_local(CarryArgs A) {
     __syncthreads();
 
     u32 f[E], incl[E];
-    FMH_GLOBAL u64* out = gmem(D.out);
+    u64 lo[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int sg = e * NW + wave;
@@ -1261,8 +1265,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_local(CarryArgs A) {
         // byte c: h + [l + c >= 2^64]
         f[e] = h * 0x01010101u + ((l == ~0ull ? 1u : 0u) << 8) + ((l >= ~0ull - 1 ? 1u : 0u) << 16) +
                ((l >= ~0ull - 2 ? 1u : 0u) << 24);
-        const u64 k = k0 + (u64)(e * kCrtThreads + tid);
-        if (k < out_len) out[k] = l;
+        lo[e] = l;
         // inclusive scan of the segment's maps in lane order
         u32 m = f[e];
 #pragma unroll
@@ -1281,13 +1284,20 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_local(CarryArgs A) {
         if (sg % NW == wave) pre[sg / NW] = run;
         run = map_compose(s_agg[sg], run);
     }
-    // carry-in of each limb as a map of the chunk's carry-in
+    // carry-in of each limb as a map of the chunk's carry-in; where that map
+    // is constant (every limb past the first whose low word cannot overflow)
+    // the carry is added here and the limb is final
     FMH_GLOBAL u8* fm = gmem(A.fmap + tile * CH);
+    FMH_GLOBAL u64* out = gmem(D.out);
 #pragma unroll
     for (int e = 0; e < E; e++) {
         u32 ex = __shfl_up(incl[e], 1, 64);
         if (lane == 0) ex = kMapId;
-        fm[e * kCrtThreads + tid] = (u8)map_pack(map_compose(ex, pre[e]));
+        const u32 mp = map_compose(ex, pre[e]);
+        const u32 pk = map_pack(mp);
+        fm[e * kCrtThreads + tid] = (u8)pk;
+        const u64 k = k0 + (u64)(e * kCrtThreads + tid);
+        if (k < out_len) out[k] = lo[e] + (map_const(pk) ? map_apply(mp, 0) : 0u);
     }
     if (tid == 0) A.chunk_map[tile] = run;
 }
@@ -1367,20 +1377,18 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     } else {
         cin = A.chunk_cin[tile];
     }
+    // limbs whose carry-in map is constant were finished by the CRT kernel
+    // (in practice all but the chunk's first)
     FMH_GLOBAL u64* out = gmem(D.out + k0);
     u32 b[E];
-    u64 w[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) b[e] = fm[e * kThreads + tid];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const u64 i = (u64)(e * kThreads + tid);
-        b[e] = fm[i];
-        w[e] = out[i < lim ? i : lim - 1];
-    }
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const u64 i = (u64)(e * kThreads + tid);
+        if (map_const(b[e]) || i >= lim) continue;
         const u32 c = map_apply(map_unpack(b[e]), cin);
-        if (c != 0 && i < lim) out[i] = w[e] + c;
+        if (c != 0) out[i] += c;
     }
 }
 

@@ -87,6 +87,13 @@ for n1, n2 in [(3000, 100), (100, 3000), (5000, 7), (2600, 300)]:
 print("unbalanced ok", flush=True)
 ones = np.full(300, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
 check(np.array_equal(mul(ones, ones[:200]), orc.mul(ones, ones[:200])), "all ones")
+# carry chains across 1024-limb chunks: runs of all-ones limbs leave the
+# carry-in maps of whole chunks non-constant (chunk scan and fused paths)
+big_ones = np.full(3000, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+check(np.array_equal(mul(big_ones, big_ones[:2500]), orc.mul(big_ones, big_ones[:2500])), "all ones, 6 chunks")
+b = np.zeros(1030, np.uint64)
+b[0] = b[-1] = 1
+check(np.array_equal(mul(big_ones[:2100], b), orc.mul(big_ones[:2100], b)), "ones x (1 + 2^(64 1029)), 4 chunks")
 
 # null pointers with nonzero lengths are refused before any launch
 L.fmh_mul_device.argtypes = [vp, u, vp, u, vp, vp]
