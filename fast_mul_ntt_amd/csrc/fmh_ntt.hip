@@ -1364,13 +1364,19 @@ __global__ __launch_bounds__(kThreads) void k_carry_apply(CarryArgs A) {
     if (FUSED) {
         // maps of chunks chunk0 .. tile - 1, thread t composing a contiguous
         // run of them in order
+        // (before < kFusedChunks: every load of a thread is issued at once)
+        constexpr u32 kPer = (kFusedChunks + kThreads - 1) / kThreads;
         __shared__ u32 s_wave[kThreads / 64];
         const u32 before = (u32)(tile - D.chunk0);
         const u32 per = (before + kThreads - 1) / kThreads;
         const u32 beg = min(before, (u32)tid * per), end = min(before, beg + per);
         const FMH_GLOBAL u32* maps = gmem(A.chunk_map + D.chunk0);
+        u32 mv[kPer];
+#pragma unroll
+        for (u32 k = 0; k < kPer; k++) mv[k] = beg + k < end ? maps[beg + k] : kMapId;
         u32 seg = kMapId;
-        for (u32 i = beg; i < end; i++) seg = map_compose(maps[i], seg);
+#pragma unroll
+        for (u32 k = 0; k < kPer; k++) seg = map_compose(mv[k], seg);
         u32 all;
         (void)block_excl_scan_maps(seg, s_wave, &all);
         cin = map_apply(all, 0);
