@@ -40,20 +40,20 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against the copy
 # kernel) from rocprofv3 --pmc runs of this workload; profiles/ holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 205333400.0,
-    "row_mul": 154585187.0,
-    "col_inv": 151204459.0,
-    "crt_local": 70275187.0,
-    "carry_apply": 34716692.0,
+    "col_fwd": 203084244.0,
+    "row_mul": 154806686.0,
+    "col_inv": 151434516.0,
+    "crt_local": 70206947.0,
+    "carry_apply": 2386745.0,
 }
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
 # the two column passes).
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3560709083.0,
-    "row_mul": 2418188376.0,
-    "col_inv": 2202057931.0,
-    "crt_local": 1121036584.0,
-    "carry_apply": 555751545.0,
+    "col_fwd": 3518029302.0,
+    "row_mul": 2419013710.0,
+    "col_inv": 2042241809.0,
+    "crt_local": 1120999807.0,
+    "carry_apply": 38289045.0,
 }
 
 
@@ -83,8 +83,9 @@ def kernel_alg_bytes(n1: int, n2: int, log_n: int, col_passes: int = 1) -> dict[
     col_inv     each pass reads and writes 3 residue arrays
     crt_local   reads 3 residues per convolution coefficient, writes the
                 limbs and one carry-map byte per limb
-    carry_apply reads the carry-map bytes and adds the carries into the limbs
-                (read-modify-write: a carry of 1 or 2 is the common case)
+    carry_apply reads the carry-map bytes, and read-modify-writes the limbs
+                whose carry-in map is not constant: for operands like these
+                (random words) only each 1024-limb chunk's first limb
     """
     N = 1 << log_n
     conv = n1 + n2 - 1
@@ -95,7 +96,7 @@ def kernel_alg_bytes(n1: int, n2: int, log_n: int, col_passes: int = 1) -> dict[
         "row_mul": 9 * 8.0 * N,
         "col_inv": 6 * 8.0 * N,
         "crt_local": 3 * 8.0 * conv + 9.0 * out,
-        "carry_apply": 17.0 * out,
+        "carry_apply": 1.0 * out + 16.0 * ((out + 1023) // 1024),
     }
 
 
