@@ -14,8 +14,10 @@
 // only combines indices that differ in bits n-s1 .. n-s0-1, so a workgroup
 // loads one "line" per (block, offset) pair: 2^(s1-s0) words at stride
 // B = 2^(n-s1), a few adjacent lines at once so every row of the tile is one
-// coalesced segment.  The group's stages then run in LDS as radix-2^2 rounds
-// (two stages per LDS round trip, the reference's radix-2^2 pairing).  The
+// coalesced segment.  The group's stages then run as radix-2^2 rounds (two
+// stages per LDS round trip, the reference's radix-2^2 pairing); the first
+// round reads its words from global memory and the last writes them there,
+// so the tile is never copied in or out on its own.  The
 // last group (stride 1) is the row pass; for products it is fused with the
 // pointwise product and the first inverse stages, so the spectra never
 // travel to HBM.

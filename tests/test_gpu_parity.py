@@ -119,9 +119,10 @@ def test_squares_vs_oracle(fm, orc, n):
     assert np.array_equal(fm.abs_sqr64(a), orc.sqr(a))
 
 
-@pytest.mark.parametrize("n1,n2", [(1, 1), (7, 3), (3000, 3000), (70000, 1000), (1 << 20, 1 << 20)])
+@pytest.mark.parametrize("n1,n2", [(1, 1), (7, 3), (3000, 3000), (70000, 1000), (1 << 20, 1 << 20), (1 << 21, 1 << 20)])
 def test_all_ones_worst_carry(fm, n1, n2):
-    """(B^a - 1)(B^b - 1) = B^(a+b) - B^a - B^b + 1 with B = 2^64: carry chains of length ~n."""
+    """(B^a - 1)(B^b - 1) = B^(a+b) - B^a - B^b + 1 with B = 2^64: carry chains of length ~n.
+    2^21 + 2^20 output limbs are 3072 carry chunks: the separate chunk-scan path."""
     a = np.full(n1, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
     b = np.full(n2, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
     got = fm.abs_mul64(a, b)
