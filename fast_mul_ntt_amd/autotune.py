@@ -47,7 +47,7 @@ import fast_mul_ntt_amd as fm  # noqa: E402
 
 CANDIDATES = {
     "row_maxl": [8, 9, 10, 11, 12],
-    "row_tile": [10, 11, 12, 13],
+    "row_tile": [9, 10, 11, 12, 13],
     "col_maxl": [7, 8, 9, 10, 11],
     "col_logc": [2, 3, 4, 5],
     "col_threads": [64, 128, 256, 512, 1024],
