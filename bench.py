@@ -30,6 +30,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from fast_mul_ntt_amd.workloads import (batch_operand, batch_products, serve_requests,  # noqa: E402
+                                        serve_source, splitmix64)
+
 METRIC = "µs per 2^20×2^20-limb abs_mul64 (bit-exact); mixed-size products/s + p99"
 METRIC_16M = "µs per 2^24×2^24-limb abs_mul64 (bit-exact)"
 N1 = N2 = 1 << 20
@@ -55,20 +58,6 @@ TRAFFIC_16M: dict[str, float] = {
     "crt_local": 1120999807.0,
     "carry_apply": 38289045.0,
 }
-
-
-def splitmix64(seed: int, n: int, top_nonzero: bool = True) -> np.ndarray:
-    """splitmix64 stream (Steele, Lea, Flood 2014): state += golden gamma, then mix."""
-    gamma = np.uint64(0x9E3779B97F4A7C15)
-    with np.errstate(over="ignore"):
-        s = np.uint64(seed) + gamma * np.arange(1, n + 1, dtype=np.uint64)
-        z = s
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    if top_nonzero and n and z[-1] == 0:
-        z[-1] = 1
-    return z
 
 
 def kernel_alg_bytes(n1: int, n2: int, log_n: int, col_passes: int = 1) -> dict[str, float]:
@@ -146,26 +135,17 @@ def cpu_fft_small(a: np.ndarray, b: np.ndarray, want: np.ndarray, budget_s: floa
     return times
 
 
-def log_uniform_lengths(rng: np.random.Generator, count: int, lo: int, hi: int) -> np.ndarray:
-    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(np.int64)
-
-
 def run_batch(args, fm) -> dict:
     """4,096 products, operand lengths log-uniform in [64, 65,536] limbs drawn
     independently, 25 % squarings (sampler seed 4001), seeded-uniform limbs:
     one planned batch (graph replay) against 4,096 plain device calls."""
-    count = 4096
-    rng = np.random.default_rng(4001)
-    l1 = log_uniform_lengths(rng, count, 64, 65536)
-    l2 = log_uniform_lengths(rng, count, 64, 65536)
-    sq = rng.random(count) < 0.25
+    l1, l2, sq = batch_products()
+    count = l1.size
     stream = fm.Stream()
     items, bufs = [], []
-    seed = 40_000
     for i in range(count):
-        da = fm.DeviceBuffer.from_host(splitmix64(seed, int(l1[i])), stream)
-        db = None if sq[i] else fm.DeviceBuffer.from_host(splitmix64(seed + 1, int(l2[i])), stream)
-        seed += 2
+        da = fm.DeviceBuffer.from_host(batch_operand(i, 0, int(l1[i])), stream)
+        db = None if sq[i] else fm.DeviceBuffer.from_host(batch_operand(i, 1, int(l2[i])), stream)
         n_out = 2 * int(l1[i]) if sq[i] else int(l1[i] + l2[i])
         do = fm.DeviceBuffer(n_out)
         bufs += [da, do] + ([db] if db is not None else [])
@@ -228,21 +208,6 @@ def run_batch(args, fm) -> dict:
     }
 
 
-def serve_requests(count: int = 20000, seed: int = 5001):
-    """Request stream: lengths log-uniform on [256, 4,194,304] limbs drawn
-    independently per operand, 20 % squarings, operand slices at seeded
-    offsets of one resident random limb array."""
-    rng = np.random.default_rng(seed)
-    l1 = log_uniform_lengths(rng, count, 256, 4194304)
-    l2 = log_uniform_lengths(rng, count, 256, 4194304)
-    sq = rng.random(count) < 0.20
-    src_len = 2 * 4194304
-    o1 = (rng.random(count) * (src_len - l1 + 1)).astype(np.int64)
-    o2 = (rng.random(count) * (src_len - l2 + 1)).astype(np.int64)
-    gaps = rng.exponential(1.0, size=count)  # unit-rate Poisson gaps, scaled by 1/lambda
-    return l1, l2, sq, o1, o2, gaps, src_len
-
-
 def run_serve(args, fm) -> dict:
     """20,000 multiply/square requests through the request loop (64 GiB pool,
     continuous batching), against plain per-request device calls and against
@@ -250,7 +215,7 @@ def run_serve(args, fm) -> dict:
     count = args.requests
     l1, l2, sq, o1, o2, gaps, src_len = serve_requests(count)
     stream = fm.Stream()
-    src_host = splitmix64(5001, src_len)
+    src_host = serve_source(src_len)
     src = fm.DeviceBuffer.from_host(src_host, stream)
     ops = []
     for i in range(count):
@@ -434,9 +399,17 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
     db = fm.DeviceBuffer.from_host(b, stream)
     dout = fm.DeviceBuffer(n1 + n2)
 
-    for _ in range(args.warmup):
-        fm.mul_device(da, n1, db, n2, dout, stream)
-    stream.synchronize()
+    # Warm-up: at least W steps and at least --min-warmup-s seconds of them,
+    # so that the device has left its idle clocks before anything is timed
+    # (a few steps take about a millisecond).
+    warm_steps = 0
+    w0 = time.perf_counter()
+    while warm_steps < args.warmup or time.perf_counter() - w0 < args.min_warmup_s:
+        for _ in range(8):
+            fm.mul_device(da, n1, db, n2, dout, stream)
+        stream.synchronize()
+        warm_steps += 8
+    warm_s = time.perf_counter() - w0
     fm.timing_collect()
 
     # Breakdown pass (not the timed region): events around every kernel.
@@ -503,6 +476,8 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": warm_steps,
+        "warmup_s": round(warm_s, 3),
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": False,
         "scaling": "weak",
@@ -575,6 +550,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--min-warmup-s", type=float, default=2.0,
+                    help="warm-up runs at least --warmup steps and at least this many seconds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB copy kernel (profiler byte-counter calibration)")

@@ -13,7 +13,7 @@ starting from the defaults (batched classes: from the class's single-product
 entry):
 
     row_maxl     points per row-pass line          2^8 .. 2^12
-    row_tile     points per row-pass tile          2^10 .. 2^13
+    row_tile     points per row-pass tile          2^9 .. 2^13
     col_maxl     points per column-pass line       2^7 .. 2^11
     col_logc     adjacent lines per column tile    4 .. 32
     col_threads  threads per column workgroup      64 .. 1024

@@ -351,7 +351,14 @@ void abs_sqr64(const u64* in1, u64 len1, u64* out) {
 
 int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
                    fmh_stream_t stream) {
-    if (len2 && !d_in2) return set_err("NULL device pointer for a non-empty operand");
+    if ((len1 && !d_in1) || (len2 && !d_in2)) return set_err("NULL device pointer for a non-empty operand");
+    if (len1 + len2 && !d_out) return set_err("NULL device pointer for a non-empty output");
+    if (len1 == 0 || len2 == 0) {
+        // the all-zero product of len1 + len2 limbs; a null operand here must
+        // not be read as the square of the other one (the batch convention)
+        hipError_t e = enqueue_zero(d_out, len1 + len2, (hipStream_t)stream);
+        return e == hipSuccess ? 0 : set_err("kernel launch", e);
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     DevCtx* c = nullptr;
     if (ctx_get(&c)) return -1;
@@ -455,6 +462,7 @@ int fmh_mul_spectrum_device(const fmh_spectrum* sp, const u64* d_in2, u64 len2, 
                             fmh_stream_t stream) {
     if (!sp) return set_err("NULL spectrum");
     if (len2 && !d_in2) return set_err("NULL operand");
+    if (!d_out) return set_err("NULL device pointer for a non-empty output");  // sp->len >= 1
     if (len2 && sp->len + len2 - 1 > (1ull << sp->log_n))
         return set_err("product longer than the spectrum's transform length");
     ProductIn in{d_in2, len2, nullptr, sp->len, d_out, sp->words, sp->log_n};

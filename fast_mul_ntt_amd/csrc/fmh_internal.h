@@ -171,7 +171,10 @@ void prepare_kernels();
 
 // Fills the job; returns 0, -1 if a product is longer than 2^kMaxLogN, or -2
 // if a product does not fit the transform length of its resident spectrum.
-int job_build(Job& j, const ProductIn* in, size_t count);
+// allow_split = false keeps every product one transform item (no pieces
+// sharing an internal spectrum), so that a single product's job needs no
+// descriptors in device memory.
+int job_build(Job& j, const ProductIn* in, size_t count, bool allow_split = true);
 // False if a non-empty operand or output of the products is a null pointer
 // (which the kernels would dereference).
 bool products_valid(const ProductIn* in, size_t count);

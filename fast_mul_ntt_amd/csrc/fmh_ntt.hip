@@ -1595,7 +1595,7 @@ bool products_valid(const ProductIn* in, size_t count) {
     return true;
 }
 
-int job_build(Job& j, const ProductIn* in, size_t count) {
+int job_build(Job& j, const ProductIn* in, size_t count, bool allow_split) {
     j = Job();
     std::vector<ProductDesc> items;
     std::vector<u32> owner, part;  // crt entry and piece of each transform item
@@ -1635,7 +1635,9 @@ int job_build(Job& j, const ProductIn* in, size_t count) {
         const u32 ci = (u32)j.crt.size();
         u64 m = 0;
         u32 K = 0;
-        const int k = d.kind == kKindMul ? split_plan(std::max(d.na, d.nb), std::min(d.na, d.nb), log_n, &m, &K) : 0;
+        const int k = allow_split && d.kind == kKindMul
+                          ? split_plan(std::max(d.na, d.nb), std::min(d.na, d.nb), log_n, &m, &K)
+                          : 0;
         if (k) {
             // long operand a, short operand b
             const u64* la = d.na >= d.nb ? d.a : d.b;

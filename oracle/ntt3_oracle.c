@@ -749,6 +749,17 @@ u64 orc_mod_q(const u64* a, u64 n, u64 q) {
 
 u64 orc_mulmod_q(u64 a, u64 b, u64 q) { return (u64)(((u128)a * b) % q); }
 
+void orc_prefix_mod_q(const u64* a, u64 n, u64 q, u64* out) {
+    const u64 b = (u64)((((u128)1) << 64) % q); /* 2^64 mod q */
+    u64 acc = 0, pw = 1 % q;                    /* pw = 2^(64 k) mod q */
+    out[0] = 0;
+    for (u64 k = 0; k < n; k++) {
+        acc = (u64)(((u128)acc + (u128)(a[k] % q) * pw) % q);
+        pw = (u64)(((u128)pw * b) % q);
+        out[k + 1] = acc;
+    }
+}
+
 void orc_splitmix64_fill(u64 seed, u64* out, u64 n, int top_nonzero) {
     u64 s = seed;
     for (u64 k = 0; k < n; k++) {

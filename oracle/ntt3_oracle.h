@@ -82,6 +82,10 @@ void orc_plant(int kind);
 void orc_schoolbook(const u64* a, u64 na, const u64* b, u64 nb, u64* out);
 u64 orc_mod_q(const u64* a, u64 n, u64 q);          /* value mod q, q < 2^63 */
 u64 orc_mulmod_q(u64 a, u64 b, u64 q);
+/* out[k] = (a[0] + a[1] 2^64 + ... + a[k-1] 2^(64 (k-1))) mod q for k = 0..n
+ * (n + 1 words): residues of every slice a[o, o + l) follow as
+ * (out[o + l] - out[o]) 2^(-64 o) mod q. */
+void orc_prefix_mod_q(const u64* a, u64 n, u64 q, u64* out);
 /* splitmix64 stream starting at state `seed`; if top_nonzero and out[n-1]==0,
  * out[n-1] is set to 1. */
 void orc_splitmix64_fill(u64 seed, u64* out, u64 n, int top_nonzero);

@@ -59,6 +59,7 @@ class Oracle:
         L.orc_mod_q.restype = _u64
         L.orc_mulmod_q.argtypes = [_u64, _u64, _u64]
         L.orc_mulmod_q.restype = _u64
+        L.orc_prefix_mod_q.argtypes = [_pu64, _u64, _u64, _pu64]
         L.orc_splitmix64_fill.argtypes = [_u64, _pu64, _u64, ctypes.c_int]
         L.orc_plant.argtypes = [ctypes.c_int]
         self.L = L
@@ -135,6 +136,19 @@ class Oracle:
 
     def mulmod_q(self, a: int, b: int, q: int) -> int:
         return int(self.L.orc_mulmod_q(a, b, q))
+
+    def prefix_mod_q(self, a, q: int) -> np.ndarray:
+        """Residues mod q of every prefix a[:k], k = 0..len(a)."""
+        a = _limbs(a)
+        out = np.zeros(a.size + 1, np.uint64)
+        self.L.orc_prefix_mod_q(_ptr(a), a.size, q, _ptr(out))
+        return out
+
+    @staticmethod
+    def slice_mod_q(prefix: np.ndarray, offset: int, n: int, q: int) -> int:
+        """Residue mod q of the number a[offset:offset + n], from prefix_mod_q(a, q)."""
+        d = (int(prefix[offset + n]) - int(prefix[offset])) % q
+        return d * pow(pow(2, 64, q), -offset, q) % q
 
     def splitmix(self, seed: int, n: int, top_nonzero: bool = True) -> np.ndarray:
         out = np.zeros(n, np.uint64)

@@ -97,6 +97,7 @@ check(np.array_equal(mul(big_ones[:2100], b), orc.mul(big_ones[:2100], b)), "one
 
 # null pointers with nonzero lengths are refused before any launch
 L.fmh_mul_device.argtypes = [vp, u, vp, u, vp, vp]
+L.fmh_sqr_device.argtypes = [vp, u, vp, vp]
 L.fmh_last_error.restype = ctypes.c_char_p
 buf = np.zeros(64, np.uint64)
 check(L.fmh_mul_device(None, 8, buf.ctypes.data, 8, buf.ctypes.data + 256, None) != 0, "null operand")
@@ -108,6 +109,21 @@ check(L.fmh_spectrum_create(None, 8, 0, None, ctypes.byref(h0)) != 0, "null spec
 buf[:] = 7
 check(L.fmh_mul_device(None, 0, buf.ctypes.data, 8, buf.ctypes.data + 256, None) == 0, "empty null operand")
 check(not buf[32:40].any(), "empty product is zero")
+# the mirrored case, with an output of exactly len1 limbs: zeros, not a square
+# of the first operand (which would write 2 len1 limbs)
+a8 = orc.splitmix(77, 8)
+out8 = np.full(8, 7, np.uint64)
+check(L.fmh_mul_device(a8.ctypes.data, 8, None, 0, out8.ctypes.data, None) == 0, "empty null second operand")
+check(not out8.any(), "a x (empty) is zero")
+check(L.fmh_mul_device(a8.ctypes.data, 8, None, 0, None, None) != 0, "null output of an empty product")
+check(L.fmh_sqr_device(buf.ctypes.data, 8, None, None) != 0, "null square output")
+check(b"NULL" in L.fmh_last_error(), "null square output message")
+# lengths beyond the largest transform are refused before any memory is read
+# (the buffer is far smaller than the lengths)
+L.fmh_max_ntt_len.restype = u
+big = (L.fmh_max_ntt_len() >> 1) + 1
+check(L.fmh_mul_device(buf.ctypes.data, big, buf.ctypes.data + 8, big, buf.ctypes.data, None) != 0, "too long")
+check(b"fmh_max_ntt_len" in L.fmh_last_error(), "too long message")
 
 # forward spectra (device pointers are host pointers here)
 for log_n in [0, 3, 13]:
@@ -138,6 +154,12 @@ for n1, ntt_len, others in [(7, 0, [1, 7]), (7, 2048, [9, 1500]), (600, 0, [500,
     big = np.ones(N, np.uint64)
     out = np.zeros(N + n1, np.uint64)
     check(L.fmh_mul_spectrum_device(h, big.ctypes.data, N, out.ctypes.data, None) != 0, ("spec too long", n1))
+    # a null output is refused, also for an empty second operand (zero product)
+    check(L.fmh_mul_spectrum_device(h, big.ctypes.data, 1, None, None) != 0, ("spec null out", n1))
+    check(L.fmh_mul_spectrum_device(h, None, 0, None, None) != 0, ("spec null out, empty", n1))
+    check(L.fmh_sqr_spectrum_device(h, None, None) != 0, ("spec sqr null out", n1))
+    out = np.full(n1, 5, np.uint64)
+    check(L.fmh_mul_spectrum_device(h, None, 0, out.ctypes.data, None) == 0 and not out.any(), ("spec x empty", n1))
     check(L.fmh_spectrum_destroy(h) == 0, ("spec destroy", n1))
 print("spectra ok", flush=True)
 

@@ -347,6 +347,23 @@ def test_mod_q_matches_python(orc):
         assert orc.mod_q(a, q) == limbs_to_int(a) % q
 
 
+def test_slice_residues_from_prefixes(orc):
+    """Residues of slices of one array from its prefix residues (the serving
+    workload's operands are slices of one source array) equal Python big-int
+    residues of the slices, and direct mod_q."""
+    rng = random.Random(9)
+    src = orc.splitmix(123, 5000)
+    for _ in range(6):
+        q = rng.getrandbits(61) | (1 << 60) | 1
+        pre = orc.prefix_mod_q(src, q)
+        assert pre[0] == 0 and int(pre[-1]) == limbs_to_int(src) % q
+        for _ in range(10):
+            o = rng.randint(0, 4999)
+            n = rng.randint(0, 5000 - o)
+            want = limbs_to_int(src[o:o + n]) % q
+            assert orc.slice_mod_q(pre, o, n, q) == want == orc.mod_q(src[o:o + n], q), (o, n)
+
+
 def test_zero_length_operands(orc):
     a = orc.splitmix(1, 4)
     assert np.array_equal(orc.mul(a, np.zeros(0, np.uint64)), np.zeros(4, np.uint64))
