@@ -32,6 +32,7 @@ __all__ = [
     "Stream",
     "mul_device",
     "sqr_device",
+    "mul_device_noalloc",
     "forward_spectrum",
     "copy_device",
     "FmhProduct",
@@ -110,6 +111,7 @@ def lib() -> ctypes.CDLL:
     L.abs_sqr64.restype = None
     L.fmh_mul_device.argtypes = [_vp, _u64, _vp, _u64, _vp, _vp]
     L.fmh_sqr_device.argtypes = [_vp, _u64, _vp, _vp]
+    L.fmh_mul_device_noalloc.argtypes = [_vp, _u64, _vp, _u64, _vp, _vp]
     L.fmh_forward_spectrum_device.argtypes = [ctypes.c_int, _vp, _u64, _u64, _vp, _vp]
     L.fmh_max_ntt_len.restype = _u64
     L.fmh_init.argtypes = [ctypes.c_int]
@@ -292,6 +294,16 @@ def mul_device(a: DeviceBuffer, na: int, b: DeviceBuffer, nb: int, out: DeviceBu
         raise ValueError("device buffers too small")
     _check(lib().fmh_mul_device(a.ptr, na, b.ptr, nb, out.ptr, stream.handle if stream else None),
            "fmh_mul_device")
+
+
+def mul_device_noalloc(a: DeviceBuffer, na: int, b: DeviceBuffer, nb: int, out: DeviceBuffer,
+                       stream: Stream | None = None) -> None:
+    """fmh_mul_device_noalloc: out = a[:na] * b[:nb] with the library's fixed
+    device scratch (no device allocation; fmh_init first)."""
+    if out.n < na + nb or a.n < na or b.n < nb:
+        raise ValueError("device buffers too small")
+    _check(lib().fmh_mul_device_noalloc(a.ptr, na, b.ptr, nb, out.ptr, stream.handle if stream else None),
+           "fmh_mul_device_noalloc")
 
 
 def sqr_device(a: DeviceBuffer, na: int, out: DeviceBuffer, stream: Stream | None = None) -> None:

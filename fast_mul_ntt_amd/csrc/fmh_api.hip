@@ -109,9 +109,25 @@ CopyPool& copy_pool() {
 #endif
 constexpr size_t kPinChunk = FMH_PIN_CHUNK;
 
+// Device scratch of fmh_mul_device_noalloc: the workspace (residues of both
+// operands for the three primes, per-limb carry maps, per-chunk maps and
+// carry-ins) of one product of at most 2^kNoAllocLogN points, laid out as
+// job_bind lays out any workspace.  Every GPU the library runs on holds its
+// own copy until the process ends.  Host code never uses the array's name as
+// an address: fmh_init looks up each device's copy with hipGetSymbolAddress
+// and keeps it in DevCtx::scratch.
+constexpr int kNoAllocLogN = 20;
+constexpr size_t align256c(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocChunks = ((1ull << kNoAllocLogN) + 1 + kCarryChunk - 1) / kCarryChunk;
+constexpr size_t kNoAllocBytes = align256c((size_t)6 * sizeof(u64) << kNoAllocLogN) +
+                                 align256c(kNoAllocChunks * kCarryChunk) + 2 * align256c(kNoAllocChunks * sizeof(u32));
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static_assert((1ull << kNoAllocLogN) == FMH_NOALLOC_MAX_NTT_LEN, "fmh.h states the limit");
+
 struct DevCtx {
     bool ready = false;
     hipStream_t stream = nullptr;  // private stream of the host-pointer API
+    void* scratch = nullptr;       // this device's copy of `scratch` (set by fmh_init)
     Twiddles tw;
     std::map<hipStream_t, Buffer> ws;
     Buffer stage_a, stage_b, stage_out;
@@ -581,8 +597,52 @@ int fmh_init(int device) {
     std::lock_guard<std::mutex> lk(g_mu);
     DevCtx* c = nullptr;
     if (ctx_get(&c)) return -1;
-    return ensure_twiddles(*c, kMinTwiddleLog);
+    if (ensure_twiddles(*c, kMinTwiddleLog)) return -1;
+    prepare_kernels();
+    if (!c->scratch) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return set_err("hipGetSymbolAddress(scratch)", e);
+        c->scratch = p;
+    }
+    return 0;
 }
+
+int fmh_mul_device_noalloc(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
+                           fmh_stream_t stream) {
+    if ((len1 && !d_in1) || (len2 && !d_in2)) return set_err("NULL device pointer for a non-empty operand");
+    if (len1 + len2 && !d_out) return set_err("NULL device pointer for a non-empty output");
+    if (len1 && len2 && len1 + len2 - 1 > (1ull << kNoAllocLogN))
+        return set_err("product longer than FMH_NOALLOC_MAX_NTT_LEN coefficients");
+    Twiddles tw;
+    void* ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return set_err("hipGetDevice", e);
+        if (dev < 0 || dev >= kMaxDevices || !g_ctx[dev].scratch)
+            return set_err("fmh_mul_device_noalloc: call fmh_init on this device first");
+        tw = g_ctx[dev].tw;  // fmh_init built tables of at least 2^kNoAllocLogN points
+        ws = g_ctx[dev].scratch;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (len1 == 0 || len2 == 0) {
+        hipError_t e = enqueue_zero(d_out, len1 + len2, s);
+        return e == hipSuccess ? 0 : set_err("kernel launch", e);
+    }
+    const ProductIn in{d_in1, len1, d_in2, len2, d_out};
+    Job job;
+    if (job_build(job, &in, 1, /*allow_split=*/false)) return set_err("product longer than fmh_max_ntt_len()");
+    // one product, one transform item: the descriptors travel as kernel
+    // arguments, and the whole workspace is the scratch array
+    if (job_desc_bytes(job) != 0 || job_ws_bytes(job) > kNoAllocBytes || job.max_log_n > tw.log_t)
+        return set_err("fmh_mul_device_noalloc: product does not fit the scratch array");
+    job_bind(job, ws);
+    hipError_t e = job_launch(job, tw, s);
+    return e == hipSuccess ? 0 : set_err("kernel launch", e);
+}
+
 
 int fmh_dev_alloc(void** ptr, u64 bytes) {
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);

@@ -44,6 +44,21 @@ int fmh_mul_device(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* 
                    fmh_stream_t stream);
 int fmh_sqr_device(const u64* d_in1, u64 len1, u64* d_out, fmh_stream_t stream);
 
+/* Allocation-free variant of fmh_mul_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc / hipFree: its workspace is a fixed device array of
+ * the library (about 49 MiB per GPU, held until the process ends), so it
+ * takes products of len1 + len2 - 1 <= FMH_NOALLOC_MAX_NTT_LEN coefficients
+ * (larger ones are refused with an error) and needs fmh_init(device) first.
+ * A square is d_in2 == d_in1, len2 == len1.  Products are never split into
+ * pieces here; results equal fmh_mul_device's.  Every call on one GPU uses
+ * the same array: calls on one stream run in order, calls that overlap in
+ * time (other streams, other threads) race and must not be made.          */
+#define FMH_NOALLOC_MAX_NTT_LEN (1ull << 20)
+int fmh_mul_device_noalloc(const u64* d_in1, u64 len1, const u64* d_in2, u64 len2, u64* d_out,
+                           fmh_stream_t stream);
+
 /* Forward transform of one operand for prime index `prime` (0, 1, 2 = the
  * reference's moduli 1, 2, 3), zero-padded to ntt_len (a power of two >=
  * len): ntt_len canonical residues X * 2^64 mod p in the reference's order,

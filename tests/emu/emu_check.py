@@ -125,6 +125,28 @@ big = (L.fmh_max_ntt_len() >> 1) + 1
 check(L.fmh_mul_device(buf.ctypes.data, big, buf.ctypes.data + 8, big, buf.ctypes.data, None) != 0, "too long")
 check(b"fmh_max_ntt_len" in L.fmh_last_error(), "too long message")
 
+# the allocation-free variant: the same products as fmh_mul_device (two calls
+# in a row, the second smaller; a square; a shape the other entry point
+# splits), and refusal of a product longer than its scratch array
+L.fmh_init.argtypes = [ctypes.c_int]
+L.fmh_mul_device_noalloc.argtypes = [vp, u, vp, u, vp, vp]
+check(L.fmh_init(0) == 0, "init")
+for n1, n2, sq in [(3000, 2001, False), (700, 100, False), (5000, 7, False), (1500, 0, True)]:
+    a = orc.splitmix(n1 + 31, n1)
+    b = a if sq else orc.splitmix(n2 + 37, n2)
+    nb = b.size
+    o1 = np.full(n1 + nb, 9, np.uint64)
+    o2 = np.full(n1 + nb, 8, np.uint64)
+    check(L.fmh_mul_device_noalloc(a.ctypes.data, n1, b.ctypes.data, nb, o1.ctypes.data, None) == 0,
+          ("noalloc rc", n1, n2, L.fmh_last_error()))
+    check(L.fmh_mul_device(a.ctypes.data, n1, b.ctypes.data, nb, o2.ctypes.data, None) == 0, ("device rc", n1))
+    check(np.array_equal(o1, o2) and np.array_equal(o1, orc.mul(a, b)), ("noalloc", n1, n2, sq))
+check(L.fmh_mul_device_noalloc(buf.ctypes.data, 1 << 19, buf.ctypes.data, (1 << 19) + 2, buf.ctypes.data,
+                               None) != 0, "noalloc too long")
+check(b"FMH_NOALLOC_MAX_NTT_LEN" in L.fmh_last_error(), "noalloc too long message")
+check(L.fmh_mul_device_noalloc(None, 8, buf.ctypes.data, 8, buf.ctypes.data, None) != 0, "noalloc null operand")
+print("noalloc ok", flush=True)
+
 # forward spectra (device pointers are host pointers here)
 for log_n in [0, 3, 13]:
     n = 1 << log_n

@@ -22,6 +22,9 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
+// a __device__ variable is a host object: its "device address" is its address
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
     unsigned x, y, z;
@@ -97,6 +100,10 @@ hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipFuncSetAttribute(const void* f, hipFuncAttribute a, int v);
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);
+    return hipSuccess;
+}
 struct hipFuncAttributes {
     size_t sharedSizeBytes;
 };
