@@ -79,35 +79,68 @@ static_assert(Prime<2>::p < (~0ull) / 4, "4p < 2^64 for the largest prime");
 //   word 0: m0 = x0;  (x - m0 p) / 2^32 = hi 2^32 + D,  D = x1 - x0 PHI
 //   word 1: m1 = D mod 2^32;  result = hi + floor(D / 2^32) - m1 PHI (+ p)
 // t = x0 (2^32 - PHI) + x1 = D + x0 2^32 is exact in 64 bits, so
-// m1 = lo(t) and e = -floor(D / 2^32) = x0 - hi(t), 0 <= e < 2^30.
+// m1 = lo(t) and e = -floor(D / 2^32) = x0 - hi(t), 0 <= e < 2^30 (PHI <
+// 2^30 for these primes).  The result is hi + p - e - m1 PHI.
+//
+// The words are formed with 32 x 32 + 64-bit multiply-adds, arranged so that
+// each addend is a whole previous result wherever possible (an addend made
+// of one 32-bit word and a zero costs a register move on gfx950):
+//   p00 = a0 b0,  p01 = a0 b1 + hi(p00)
+//   q   = a1 b0 + p01 = (a1 b0 + lo(p01)) + hi(p01) 2^32, carry c out of 64 bits:
+//         lo(q) = x1, and hi(p01) + hi(a1 b0 + lo(p01)) = hi(q) + c 2^32
+//   hi  = a1 b1 + hi(p01) + hi(a1 b0 + lo(p01)) = a1 b1 + hi(q) + c 2^32
+//   u   = x0 NPHI + q (mod 2^64) = t + hi(q) 2^32: lo(u) = m1, hi(u) = hi(t) + hi(q)
+//   f   = 1 - e = hi(u) - (x0 + hi(q) - 1)  (mod 2^32, as a signed word in (-2^30, 1])
+//   acc = a1 b1 + (hi(q) + (PHI + c) 2^32) + f = hi + p - e
+//   result = acc - m1 PHI = (acc - m1 2^32) + m1 (2^32 - PHI)  (mod 2^64)
+// The host build runs the same steps in plain C++.
 template <int P>
 FMH_HD u64 redc(u64 a, u64 b) {
     constexpr u32 PHI = (u32)(Prime<P>::p >> 32);
     constexpr u32 NPHI = 0u - PHI;  // 2^32 - PHI
     static_assert(((u64)PHI << 32) + 1 == Prime<P>::p, "p = PHI 2^32 + 1");
+    static_assert(PHI < (1u << 30), "e < 2^30");
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
     const u64 p00 = (u64)a0 * b0;
     const u64 p01 = (u64)a0 * b1 + (p00 >> 32);
-    const u64 p10 = (u64)a1 * b0 + (u32)p01;
-    const u64 hi = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);
-    const u32 x0 = (u32)p00, x1 = (u32)p10;
-    const u64 t = (u64)x0 * NPHI + x1;
-    const u32 m1 = (u32)t;
-    const u32 e = x0 - (u32)(t >> 32);
-    const u64 acc = hi + (Prime<P>::p - e);
-    // result = acc - m1 PHI = (acc - m1 2^32) + m1 (2^32 - PHI)  (mod 2^64)
+    const u32 x0 = (u32)p00;
 #if defined(__HIP_DEVICE_COMPILE__)
-    // Written out so that the subtraction stays one 32-bit op on the high
-    // word and the last step one multiply-add (the compiler otherwise widens
-    // both into 64-bit borrow chains).
+    // Written out so that each multiply-add takes a whole 64-bit addend and
+    // the carry c goes straight into the next word (the compiler otherwise
+    // splits the sums into 64-bit add chains).  The unused carry outputs go
+    // to vcc, so no SGPRs are taken for them.
+    u64 q, u, acc, r;
+    u32 cp;  // PHI + c
+    // the carry-in read of c (vcc) waits two states after the write
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\ts_nop 1\n\tv_addc_co_u32_e32 %1, vcc, 0, %5, vcc"
+        : "=&v"(q), "=v"(cp)
+        : "v"(a1), "v"(b0), "v"(p01), "v"(PHI)
+        : "vcc");
+    const u32 hq = (u32)(q >> 32);
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(u) : "v"(x0), "s"(NPHI), "v"(q) : "vcc");
+    const u32 m1 = (u32)u;
+    u32 s, f, fh;
+    asm("v_add3_u32 %0, %1, %2, -1" : "=v"(s) : "v"(x0), "v"(hq));
+    asm("v_sub_u32 %0, %1, %2" : "=v"(f) : "v"((u32)(u >> 32)), "v"(s));
+    asm("v_ashrrev_i32 %0, 31, %1" : "=v"(fh) : "v"(f));
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(acc) : "v"(a1), "v"(b1), "v"(((u64)cp << 32) | hq) : "vcc");
+    acc += ((u64)fh << 32) | f;
     u32 acc_hi;
     asm("v_sub_u32 %0, %1, %2" : "=v"(acc_hi) : "v"((u32)(acc >> 32)), "v"(m1));
-    const u64 acc2 = ((u64)acc_hi << 32) | (u32)acc;
-    u64 r, carry;
-    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(m1), "s"(NPHI), "v"(acc2));
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(m1), "s"(NPHI), "v"(((u64)acc_hi << 32) | (u32)acc) : "vcc");
     return r;
 #else
-    return acc - ((u64)m1 << 32) + (u64)m1 * NPHI;
+    const u64 lo10 = (u64)a1 * b0;
+    const u64 q = lo10 + p01;
+    const u32 c = q < p01 ? 1u : 0u;
+    const u32 hq = (u32)(q >> 32);
+    const u64 u = (u64)x0 * NPHI + q;
+    const u32 m1 = (u32)u;
+    const u32 f = (u32)(u >> 32) - (x0 + hq - 1u);
+    const u64 fs = (u64)(long long)(int)f;
+    u64 acc = (u64)a1 * b1 + ((((u64)(PHI + c)) << 32) | hq) + fs;
+    acc -= (u64)m1 << 32;
+    return acc + (u64)m1 * NPHI;
 #endif
 }
 

@@ -509,14 +509,8 @@ __device__ __forceinline__ u32 xcd_slot(u32 x, u32 n) {
 }
 
 template <int P, bool FWD>
-__device__ void col_body(const ColArgs& A) {
+__device__ void col_body(const ColArgs& A, u32 tile, int op) {
     extern __shared__ u64 lds[];
-    // slot = tile * nops + operand: the operands of a tile share its twiddle
-    // rows, and neighbouring tiles share 128-byte lines of data and twiddles
-    // (a tile row is 2^logC words), so both stay in one L2.
-    const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
-    const u32 tile = slot / (u32)A.nops;
-    const int op = (int)(slot - tile * (u32)A.nops);
     const u32 prod = blockIdx.z;
     const ProductDesc& D = product(A.items, A.one, prod);
     const int logL = A.s1 - A.s0;
@@ -552,12 +546,23 @@ __device__ void col_body(const ColArgs& A) {
     }
 }
 
+// slot = (tile * 3 + prime) * nops + operand, placed by xcd_slot: the
+// operands of a tile share its twiddle rows, the three primes of a tile read
+// the same limbs (first forward pass), and neighbouring tiles share 128-byte
+// lines of data and twiddles (a tile row is 2^logC words); all of them run
+// on one XCD at about the same time, so the shared reads hit its L2.
 template <bool FWD>
 __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
-    switch (blockIdx.y) {
-        case 0: col_body<0, FWD>(A); break;
-        case 1: col_body<1, FWD>(A); break;
-        default: col_body<2, FWD>(A); break;
+    const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
+    const u32 per_tile = 3u * (u32)A.nops;
+    const u32 tile = slot / per_tile;
+    const u32 rem = slot - tile * per_tile;
+    const u32 prime = rem / (u32)A.nops;
+    const int op = (int)(rem - prime * (u32)A.nops);
+    switch (prime) {
+        case 0: col_body<0, FWD>(A, tile, op); break;
+        case 1: col_body<1, FWD>(A, tile, op); break;
+        default: col_body<2, FWD>(A, tile, op); break;
     }
 }
 
@@ -878,7 +883,7 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
 // operands, more registers; 3: radix-8 rounds, fewer registers and more
 // waves per SIMD).
 template <int P, int RMAX>
-__device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
+__device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 tile) {
     extern __shared__ u64 lds[];
     const int logL = A.log_n - A.s0;
     const int logC = A.logC;
@@ -890,7 +895,7 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
     u64* __restrict__ tA = lds;
     u64* __restrict__ tB = lds + nl;
     if ((int)threadIdx.x < C) {
-        const u64 lambda = ((u64)blockIdx.x << logC) + threadIdx.x;
+        const u64 lambda = ((u64)tile << logC) + threadIdx.x;
         const u64 prod = lambda >> A.s0;
         RowLine li{};
         li.active = prod < A.count ? 1u : 0u;
@@ -1032,13 +1037,18 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line) {
     }
 }
 
+// slot = tile * 3 + prime, placed by xcd_slot: the three primes of a tile
+// run on one XCD at about the same time, so a pass that starts from limbs
+// (small products) reads them from HBM once.
 template <int RMAX>
 __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
     __shared__ RowLine s_line[kMaxRowLines];
-    switch (blockIdx.y) {
-        case 0: row_body<0, RMAX>(A, s_line); break;
-        case 1: row_body<1, RMAX>(A, s_line); break;
-        default: row_body<2, RMAX>(A, s_line); break;
+    const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
+    const u32 tile = slot / 3u;
+    switch (slot - 3u * tile) {
+        case 0: row_body<0, RMAX>(A, s_line, tile); break;
+        case 1: row_body<1, RMAX>(A, s_line, tile); break;
+        default: row_body<2, RMAX>(A, s_line, tile); break;
     }
 }
 
@@ -1805,7 +1815,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
         if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * nops), 3, count), dim3(tu.col_threads), lds,
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3 * nops), 1, count), dim3(tu.col_threads), lds,
                    s, C);
     }
     {
@@ -1835,9 +1845,9 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
                            sizeof(u64);  // row tiles take at least 16 words (row_swz)
         if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
         if (tu.row_radix == 3)
-            FMH_LAUNCH(kRowName[kind], k_row<3>, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
+            FMH_LAUNCH(kRowName[kind], k_row<3>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.row_threads), lds, s, R);
         else
-            FMH_LAUNCH(kRowName[kind], k_row<4>, dim3((unsigned)tiles, 3, 1), dim3(tu.row_threads), lds, s, R);
+            FMH_LAUNCH(kRowName[kind], k_row<4>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.row_threads), lds, s, R);
     }
     for (int pi = pl.n_col - 1; kind != kKindSpecFwd && pi >= 0; pi--) {
         ColArgs C{};
@@ -1855,7 +1865,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
         if (lds > g_max_dyn[1]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)tiles, 3, count), dim3(tu.col_threads), lds, s, C);
+        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)(tiles * 3), 1, count), dim3(tu.col_threads), lds, s, C);
     }
     return hipSuccess;
 }
@@ -1950,7 +1960,7 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
         if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)tiles, 3, 1), dim3(tu.col_threads), lds, s, C);
+        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds, s, C);
     }
     RowArgs R{};
     R.one = d;
@@ -1974,10 +1984,10 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     const size_t lds = std::max<size_t>(16, (size_t)1 << (logL + logC)) * sizeof(u64);
     if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
     if (tu.row_radix == 3)
-        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<3>, dim3((unsigned)tiles, 3, 1),
+        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<3>, dim3((unsigned)(tiles * 3), 1, 1),
                    dim3(tu.row_threads), lds, s, R);
     else
-        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<4>, dim3((unsigned)tiles, 3, 1),
+        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<4>, dim3((unsigned)(tiles * 3), 1, 1),
                    dim3(tu.row_threads), lds, s, R);
     return hipSuccess;
 }
