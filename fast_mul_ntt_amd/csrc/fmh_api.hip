@@ -1,11 +1,12 @@
 // fmh_api.hip -- host layer and C-ABI of libfmh.so (declared in include/fmh.h).
 //
 // State per HIP device: the twiddle tables (built once on the device, grown
-// when a longer transform arrives), one scratch workspace per stream, and a
-// private stream plus staging buffers for the host-pointer drop-in.  One
-// mutex guards the state; the drop-in entry points hold it for the whole
-// call, so concurrent callers are serialised exactly as the reference's
-// reentrant functions would behave.
+// when a longer transform arrives), one scratch workspace per stream, and
+// slots of the host-pointer drop-in (a stream, device staging and a pinned
+// ring each).  One mutex guards the shared state.  A drop-in call holds a
+// slot of its own and takes the mutex only to enqueue its kernels, so
+// concurrent callers overlap their transfers and kernels, as the reference's
+// reentrant functions run concurrently on the CPU.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -53,6 +54,7 @@ class CopyPool {
             memcpy(dst, src, n);
             return;
         }
+        std::lock_guard<std::mutex> one(call_mu_);  // one split copy at a time
         std::unique_lock<std::mutex> lk(mu_);
         dst_ = (char*)dst;
         src_ = (const char*)src;
@@ -87,7 +89,7 @@ class CopyPool {
         }
     }
     std::vector<std::thread> th_;
-    std::mutex mu_;
+    std::mutex call_mu_, mu_;
     std::condition_variable cv_, done_;
     char* dst_ = nullptr;
     const char* src_ = nullptr;
@@ -102,12 +104,13 @@ CopyPool& copy_pool() {
     return pool;
 }
 
-// Two pinned staging buffers of the host-pointer calls; event i marks the
-// last transfer that used buffer i.
+// Host-pointer calls move data through a ring of kRing pinned buffers of
+// kPinChunk bytes; event i marks the last transfer that used buffer i.
 #ifndef FMH_PIN_CHUNK
 #define FMH_PIN_CHUNK (4u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
 #endif
 constexpr size_t kPinChunk = FMH_PIN_CHUNK;
+constexpr int kRing = 3;  // 4 MiB DMA chunks run at 51 GB/s, 2 MiB ones at 45 GB/s
 
 // Device scratch of fmh_mul_device_noalloc: the workspace (residues of both
 // operands for the three primes, per-limb carry maps, per-chunk maps and
@@ -124,15 +127,24 @@ constexpr size_t kNoAllocBytes = align256c((size_t)6 * sizeof(u64) << kNoAllocLo
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 static_assert((1ull << kNoAllocLogN) == FMH_NOALLOC_MAX_NTT_LEN, "fmh.h states the limit");
 
+// What one host-pointer call uses on its own: a stream, device copies of
+// the operands and the output, and the pinned ring.  Calls take a free slot
+// of their device (or make one), so concurrent callers run their transfers
+// and kernels side by side instead of one after the other.
+struct HostSlot {
+    hipStream_t stream = nullptr;
+    Buffer stage_a, stage_b, stage_out;
+    void* pin[kRing] = {};
+    hipEvent_t pin_ev[kRing] = {};
+};
+
 struct DevCtx {
     bool ready = false;
-    hipStream_t stream = nullptr;  // private stream of the host-pointer API
+    hipStream_t stream = nullptr;  // private stream of set-up work (twiddles, batch descriptors)
     void* scratch = nullptr;       // this device's copy of `scratch` (set by fmh_init)
     Twiddles tw;
     std::map<hipStream_t, Buffer> ws;
-    Buffer stage_a, stage_b, stage_out;
-    void* pin[2] = {nullptr, nullptr};
-    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    std::vector<HostSlot*> free_slots;  // host-pointer call slots not in use
 };
 
 constexpr int kMaxDevices = 64;
@@ -260,53 +272,65 @@ int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool squa
     return run_job(c, &in, 1, s);
 }
 
-int ensure_pinned(DevCtx& c) {
-    for (int i = 0; i < 2; i++) {
-        if (c.pin[i]) continue;
-        hipError_t e = hipHostMalloc(&c.pin[i], kPinChunk, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c.pin_ev[i], hipEventDisableTiming);
+int ensure_pinned(HostSlot& h) {
+    for (int i = 0; i < kRing; i++) {
+        if (h.pin[i]) continue;
+        hipError_t e = hipHostMalloc(&h.pin[i], kPinChunk, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h.pin_ev[i], hipEventDisableTiming);
         if (e != hipSuccess) return set_err("pinned staging", e);
     }
     return 0;
 }
 
-// Pageable host -> device through the pinned buffers (a pageable
-// hipMemcpy costs hundreds of microseconds even for small buffers): while
-// chunk k moves over DMA, the host threads copy chunk k + 1 into the other
-// buffer.
-int h2d_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
-    if (ensure_pinned(c)) return -1;
-    for (size_t off = 0, i = 0; off < bytes; off += kPinChunk, i ^= 1) {
-        const size_t n = std::min(kPinChunk, bytes - off);
-        hipError_t e = hipEventSynchronize(c.pin_ev[i]);
-        if (e != hipSuccess) return set_err("hipEventSynchronize", e);
-        copy_pool().copy(c.pin[i], (const char*)src + off, n);
-        e = hipMemcpyAsync((char*)dst + off, c.pin[i], n, hipMemcpyHostToDevice, c.stream);
-        if (e == hipSuccess) e = hipEventRecord(c.pin_ev[i], c.stream);
-        if (e != hipSuccess) return set_err("hipMemcpyAsync H2D", e);
-    }
+// One transfer of a staged copy: bytes between host memory and device memory.
+struct Segment {
+    void* dev;
+    void* host;
+    size_t bytes;
+};
+
+// Pageable host -> device through the pinned ring (a pageable hipMemcpy
+// costs hundreds of microseconds even for small buffers), every segment in
+// one pipeline: the host threads fill buffer k % kRing while the chunks
+// before it move over DMA, so host copies and DMA overlap across segments.
+int h2d_staged(HostSlot& h, const Segment* seg, int nseg) {
+    if (ensure_pinned(h)) return -1;
+    size_t k = 0;
+    for (int g = 0; g < nseg; g++)
+        for (size_t off = 0; off < seg[g].bytes; off += kPinChunk, k++) {
+            const size_t n = std::min(kPinChunk, seg[g].bytes - off);
+            const int i = (int)(k % kRing);
+            hipError_t e = hipEventSynchronize(h.pin_ev[i]);  // the DMA that last read buffer i
+            if (e != hipSuccess) return set_err("hipEventSynchronize", e);
+            copy_pool().copy(h.pin[i], (const char*)seg[g].host + off, n);
+            e = hipMemcpyAsync((char*)seg[g].dev + off, h.pin[i], n, hipMemcpyHostToDevice, h.stream);
+            if (e == hipSuccess) e = hipEventRecord(h.pin_ev[i], h.stream);
+            if (e != hipSuccess) return set_err("hipMemcpyAsync H2D", e);
+        }
     return 0;
 }
 
-// Device -> pageable host: chunk k + 1 is in flight over DMA while the host
-// threads copy chunk k out of its buffer.  Returns after the last byte.
-int d2h_staged(DevCtx& c, void* dst, const void* src, size_t bytes) {
-    hipError_t e;
-    if (ensure_pinned(c)) return -1;
+// Device -> pageable host: up to kRing chunks are in flight over DMA while
+// the host threads copy the oldest one out of its buffer; each buffer's next
+// DMA is enqueued as soon as it has been emptied.  Returns after the last
+// byte.
+int d2h_staged(HostSlot& h, void* dst, const void* src, size_t bytes) {
+    if (ensure_pinned(h)) return -1;
     const size_t chunks = (bytes + kPinChunk - 1) / kPinChunk;
     auto issue = [&](size_t k) {
         const size_t off = k * kPinChunk, n = std::min(kPinChunk, bytes - off);
-        hipError_t r = hipMemcpyAsync(c.pin[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, c.stream);
-        if (r == hipSuccess) r = hipEventRecord(c.pin_ev[k & 1], c.stream);
+        hipError_t r = hipMemcpyAsync(h.pin[k % kRing], (const char*)src + off, n, hipMemcpyDeviceToHost, h.stream);
+        if (r == hipSuccess) r = hipEventRecord(h.pin_ev[k % kRing], h.stream);
         return r;
     };
-    e = issue(0);
+    hipError_t e = hipSuccess;
+    for (size_t k = 0; e == hipSuccess && k < std::min(chunks, (size_t)kRing); k++) e = issue(k);
     for (size_t k = 0; e == hipSuccess && k < chunks; k++) {
-        if (k + 1 < chunks) e = issue(k + 1);  // its buffer was emptied by the copy of chunk k - 1
-        if (e == hipSuccess) e = hipEventSynchronize(c.pin_ev[k & 1]);
+        e = hipEventSynchronize(h.pin_ev[k % kRing]);
         if (e != hipSuccess) break;
         const size_t off = k * kPinChunk;
-        copy_pool().copy((char*)dst + off, c.pin[k & 1], std::min(kPinChunk, bytes - off));
+        copy_pool().copy((char*)dst + off, h.pin[k % kRing], std::min(kPinChunk, bytes - off));
+        if (k + kRing < chunks) e = issue(k + kRing);
     }
     return e == hipSuccess ? 0 : set_err("staged D2H", e);
 }
@@ -323,19 +347,42 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
         if (out_len) memset(out, 0, out_len * sizeof(u64));
         return;
     }
-    std::lock_guard<std::mutex> lk(g_mu);
+    // a slot of this device, taken from the free list or made
     DevCtx* c = nullptr;
-    if (ctx_get(&c)) die(fn);
-    if (ensure_buffer(c->stage_a, len1 * sizeof(u64), c->stream)) die(fn);
-    if (!square && ensure_buffer(c->stage_b, len2 * sizeof(u64), c->stream)) die(fn);
-    if (ensure_buffer(c->stage_out, out_len * sizeof(u64), c->stream)) die(fn);
-    u64* da = (u64*)c->stage_a.ptr;
-    u64* db = square ? da : (u64*)c->stage_b.ptr;
-    u64* dout = (u64*)c->stage_out.ptr;
-    if (h2d_staged(*c, da, in1, len1 * sizeof(u64))) die(fn);
-    if (!square && h2d_staged(*c, db, in2, len2 * sizeof(u64))) die(fn);
-    if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, c->stream)) die(fn);
-    if (d2h_staged(*c, out, dout, out_len * sizeof(u64))) die(fn);
+    HostSlot* h = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (ctx_get(&c)) die(fn);
+        if (!c->free_slots.empty()) {
+            h = c->free_slots.back();
+            c->free_slots.pop_back();
+        }
+    }
+    if (!h) {
+        h = new HostSlot();
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_err("hipStreamCreate", e);
+            die(fn);
+        }
+    }
+    // the slot's own buffers: no lock while data moves
+    if (ensure_buffer(h->stage_a, len1 * sizeof(u64), h->stream)) die(fn);
+    if (!square && ensure_buffer(h->stage_b, len2 * sizeof(u64), h->stream)) die(fn);
+    if (ensure_buffer(h->stage_out, out_len * sizeof(u64), h->stream)) die(fn);
+    u64* da = (u64*)h->stage_a.ptr;
+    u64* db = square ? da : (u64*)h->stage_b.ptr;
+    u64* dout = (u64*)h->stage_out.ptr;
+    // both operands are read before out is written, so out may alias them
+    const Segment seg[2] = {{da, (void*)in1, len1 * sizeof(u64)}, {db, (void*)in2, square ? 0 : len2 * sizeof(u64)}};
+    if (h2d_staged(*h, seg, 2)) die(fn);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);  // the device state: twiddles, per-stream workspaces
+        if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, h->stream)) die(fn);
+    }
+    if (d2h_staged(*h, out, dout, out_len * sizeof(u64))) die(fn);
+    std::lock_guard<std::mutex> lk(g_mu);
+    c->free_slots.push_back(h);
 }
 
 }  // namespace
@@ -704,10 +751,11 @@ int fmh_trim(void) {
     for (auto& kv : c.ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     c.ws.clear();
-    for (Buffer* b : {&c.stage_a, &c.stage_b, &c.stage_out}) {
-        if (b->ptr) (void)hipFree(b->ptr);
-        *b = Buffer();
-    }
+    for (HostSlot* h : c.free_slots)  // slots in use keep their buffers
+        for (Buffer* b : {&h->stage_a, &h->stage_b, &h->stage_out}) {
+            if (b->ptr) (void)hipFree(b->ptr);
+            *b = Buffer();
+        }
     return 0;
 }
 

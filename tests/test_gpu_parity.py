@@ -413,3 +413,31 @@ def test_largest_products_mod_q(fm, orc, n1, n2):
     assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFFFFFFFFFF
     for q in _primes61(3, n1 + n2):
         assert orc.mod_q(got, q) == orc.mulmod_q(orc.mod_q(a, q), orc.mod_q(b, q), q), q
+
+
+def test_concurrent_dropin_calls(fm, orc):
+    """abs_mul64 / abs_sqr64 called from several host threads at once (each
+    call takes a slot of its own; the reference's functions are reentrant,
+    src/fast_mul.c:308-460): every result equals the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+    cases = [(n1, n2, n1 * 13 + n2) for n1, n2 in
+             [(1 << 20, 1 << 20), (300000, 5000), (70000, 70000), (3000, 2500), (1 << 19, 0), (100000, 1),
+              (262144, 262144), (5000, 0)]]
+    want = {}
+    for n1, n2, seed in cases:
+        a = orc.splitmix(seed, n1)
+        want[seed] = (a, None, orc.sqr(a)) if n2 == 0 else (a, orc.splitmix(seed + 1, n2), None)
+        if n2:
+            want[seed] = (a, want[seed][1], orc.mul(a, want[seed][1]))
+
+    def call(seed):
+        a, b, w = want[seed]
+        for _ in range(3):
+            got = fm.abs_sqr64(a) if b is None else fm.abs_mul64(a, b)
+            if not np.array_equal(got, w):
+                return seed
+        return None
+
+    with ThreadPoolExecutor(max_workers=len(cases)) as ex:
+        bad = [r for r in ex.map(call, [seed for *_, seed in cases]) if r is not None]
+    assert not bad, bad
