@@ -16,6 +16,22 @@ namespace fmh {
 // 2^27 + 2^27 limbs).  The reference's own limit is 2^55 (src/data.h:69).
 constexpr int kMaxLogN = 28;
 
+// LDS of the pass kernels.  gfx950 gives a workgroup 160 KiB, static LDS
+// included.  The tuning clamp keeps every tile within the bounds below, and
+// static_asserts beside the kernels check that bound plus each kernel's
+// static LDS against kLdsPerGroup (a CPU test checks the compiled static
+// sizes as well); a launch over the limit the runtime grants is refused.
+constexpr size_t kLdsPerGroup = 160 * 1024;
+constexpr int kMaxColTileLog = 14;  // words per column tile: 2^(col_maxl + col_logc) <= 2^14
+constexpr int kMaxRowTileLog = 13;  // points per operand row tile: 2^row_tile <= 2^13
+// Dynamic LDS of a column tile of 2^logC lines of 2^logL words, and of a row
+// tile of `operands` tiles of 2^logL x 2^logC words (at least 16 words each:
+// the row-tile swizzle permutes 16-word blocks).
+constexpr size_t col_lds_bytes(int logL, int logC) { return sizeof(unsigned long long) << (logL + logC); }
+constexpr size_t row_lds_bytes(int operands, int logL, int logC) {
+    return (size_t)operands * sizeof(unsigned long long) * (logL + logC < 4 ? 16 : (size_t)1 << (logL + logC));
+}
+
 constexpr int kThreads = 256;         // threads per workgroup of the carry kernels
 constexpr int kScanThreads = 512;     // threads of the chunk scan (one workgroup per product)
 constexpr int kMaxPassThreads = 1024; // launch bound of the NTT pass kernels

@@ -126,9 +126,9 @@ static bool parse_tuning(const std::string& spec, Tuning& t) {
 static Tuning clamp_tuning(Tuning t) {
     t.col_logc = std::max(0, std::min(t.col_logc, 5));
     t.col_maxl = std::max(4, std::min(t.col_maxl, 13));  // <= 7 column passes (Plan::col_s)
-    t.row_tile = std::max(2, std::min(t.row_tile, 13));
+    t.row_tile = std::max(2, std::min(t.row_tile, kMaxRowTileLog));
     t.row_maxl = std::max(t.col_logc, std::min(t.row_maxl, t.row_tile));
-    if (t.col_maxl + t.col_logc > 14) t.col_maxl = 14 - t.col_logc;  // 128 KiB of LDS
+    if (t.col_maxl + t.col_logc > kMaxColTileLog) t.col_maxl = kMaxColTileLog - t.col_logc;
     t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
     t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
     t.row_radix = t.row_radix == 3 ? 3 : 4;
@@ -1037,6 +1037,13 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
     }
 }
 
+// Every tile the tuning clamp allows fits a workgroup's LDS together with the
+// kernels' static LDS (the column kernel has none; the row kernel has the line
+// table).
+static_assert(col_lds_bytes(kMaxColTileLog, 0) <= kLdsPerGroup, "column tile LDS");
+static_assert(row_lds_bytes(2, kMaxRowTileLog, 0) + sizeof(RowLine) * kMaxRowLines <= kLdsPerGroup,
+              "row tile LDS");
+
 // slot = tile * 3 + prime, placed by xcd_slot: the three primes of a tile
 // run on one XCD at about the same time, so a pass that starts from limbs
 // (small products) reads them from HBM once.
@@ -1542,7 +1549,7 @@ static const CrtConsts& crt_table(int log_n) {
 static size_t g_max_dyn[4] = {0, 0, 0, 0};  // k_col<true>, k_col<false>, k_row<4>, k_row<3>
 
 static size_t raise_lds_limit(const void* f) {
-    constexpr size_t kLdsPerGroup = 160 * 1024, kDefault = 64 * 1024;
+    constexpr size_t kDefault = 64 * 1024;
     hipFuncAttributes attr{};
     size_t stat = 0;
     if (hipFuncGetAttributes(&attr, f) == hipSuccess) stat = attr.sharedSizeBytes;
@@ -1813,7 +1820,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
-        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        const size_t lds = col_lds_bytes(logL, C.logC);
         if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
         FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3 * nops), 1, count), dim3(tu.col_threads), lds,
                    s, C);
@@ -1841,8 +1848,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         while (logC > 0 && (1ull << (logC - 1)) >= lines) logC--;
         R.logC = logC;
         const u64 tiles = (lines + (1ull << logC) - 1) >> logC;
-        const size_t lds = (size_t)(kind == kKindMul ? 2 : 1) * std::max<size_t>(16, (size_t)1 << (logL + logC)) *
-                           sizeof(u64);  // row tiles take at least 16 words (row_swz)
+        const size_t lds = row_lds_bytes(kind == kKindMul ? 2 : 1, logL, logC);
         if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
         if (tu.row_radix == 3)
             FMH_LAUNCH(kRowName[kind], k_row<3>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.row_threads), lds, s, R);
@@ -1863,7 +1869,7 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
-        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        const size_t lds = col_lds_bytes(logL, C.logC);
         if (lds > g_max_dyn[1]) return hipErrorInvalidValue;
         FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)(tiles * 3), 1, count), dim3(tu.col_threads), lds, s, C);
     }
@@ -1958,7 +1964,7 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
         const int logL = C.s1 - C.s0;
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
-        const size_t lds = ((size_t)1 << (logL + C.logC)) * sizeof(u64);
+        const size_t lds = col_lds_bytes(logL, C.logC);
         if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
         FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds, s, C);
     }
@@ -1981,7 +1987,7 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     const int logC = std::max(0, std::min({pl.row_s, tu.row_tile - logL, 6}));
     R.logC = logC;
     const u64 tiles = N >> (logL + logC);
-    const size_t lds = std::max<size_t>(16, (size_t)1 << (logL + logC)) * sizeof(u64);
+    const size_t lds = row_lds_bytes(1, logL, logC);
     if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
     if (tu.row_radix == 3)
         FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<3>, dim3((unsigned)(tiles * 3), 1, 1),
