@@ -289,10 +289,20 @@ def run_serve(args, fm) -> dict:
 
     drive(None)  # closed loop: capacity
     cap = srv.stats()
+    cap_classes = srv.class_report()
     rate = args.load * cap["products_per_s"]
     late = drive(rate)
     st = srv.stats()
+    classes = srv.class_report()
     srv.close()
+
+    def top_classes(rep, k=8):
+        kinds = {0: "mul", 1: "sqr", 2: "mul_spec", 3: "sqr_spec", 4: "split"}
+        tot = sum(c["device_ms"] for c in rep) or 1.0
+        rep = sorted(rep, key=lambda c: -c["device_ms"])[:k]
+        return [{"class": f"{kinds.get(c['kind'], c['kind'])} 2^{c['log_n']}", "requests": c["requests"],
+                 "groups": c["groups"], "device_ms": round(c["device_ms"], 1),
+                 "share": round(c["device_ms"] / tot, 3)} for c in rep]
     # served outputs equal the products computed alone
     for k, got in checked.items():
         plain(k)
@@ -323,6 +333,11 @@ def run_serve(args, fm) -> dict:
         "capacity_vs_plain_host_abi": round(cap["products_per_s"] / host_rate, 2),
         "mean_group": round(st["mean_group"], 2),
         "groups": st["groups"],
+        "capacity_mean_group": round(cap["mean_group"], 2),
+        "device_busy_frac": round(st["device_busy_s"] / st["span_s"], 3) if st["span_s"] else None,
+        "capacity_device_busy_frac": round(cap["device_busy_s"] / cap["span_s"], 3) if cap["span_s"] else None,
+        "capacity_classes_by_device_time": top_classes(cap_classes),
+        "classes_by_device_time": top_classes(classes),
         "pool_gib": args.pool_gib,
         "pool_peak_gib": round(st["pool_peak_bytes"] / 2**30, 3),
         "submit_lateness_p99_ms": round(float(np.percentile(late, 99)) * 1e3, 3),

@@ -83,7 +83,7 @@ class ServerInfo(ctypes.Structure):
                 ("mean_group", ctypes.c_double), ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
                 ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double),
                 ("products_per_s", ctypes.c_double), ("span_s", ctypes.c_double),
-                ("pool_bytes", _u64), ("pool_peak_bytes", _u64)]
+                ("pool_bytes", _u64), ("pool_peak_bytes", _u64), ("device_busy_s", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -146,6 +146,8 @@ def lib() -> ctypes.CDLL:
     L.fmh_server_drain.argtypes = [_vp]
     L.fmh_server_stats.argtypes = [_vp, ctypes.POINTER(ServerInfo)]
     L.fmh_server_reset_stats.argtypes = [_vp]
+    L.fmh_server_class_report.argtypes = [_vp, ctypes.c_char_p, _u64]
+    L.fmh_server_class_report.restype = _u64
     L.fmh_server_destroy.argtypes = [_vp]
     L.fmh_server_last_error.restype = ctypes.c_char_p
     L.fmh_last_error.restype = ctypes.c_char_p
@@ -487,6 +489,18 @@ class Server:
         st = ServerInfo()
         _check_srv(lib().fmh_server_stats(self.handle, ctypes.byref(st)), "fmh_server_stats")
         return st.as_dict()
+
+    def class_report(self) -> list[dict]:
+        """Device time per size class: [{kind, log_n, groups, requests, device_ms}]."""
+        n = lib().fmh_server_class_report(self.handle, None, 0)
+        buf = ctypes.create_string_buffer(int(n))
+        lib().fmh_server_class_report(self.handle, buf, len(buf))
+        out = []
+        for line in buf.value.decode().splitlines():
+            k, lg, g, r, ms = line.split()
+            out.append({"kind": int(k), "log_n": int(lg), "groups": int(g), "requests": int(r),
+                        "device_ms": float(ms)})
+        return out
 
     def reset_stats(self) -> None:
         _check_srv(lib().fmh_server_reset_stats(self.handle), "fmh_server_reset_stats")

@@ -12,9 +12,11 @@
 
 namespace fmh {
 
-// Largest transform length handled (2^28 points per prime: products of up to
-// 2^27 + 2^27 limbs).  The reference's own limit is 2^55 (src/data.h:69).
-constexpr int kMaxLogN = 28;
+// Largest transform length handled (2^30 points per prime: products of up to
+// 2^29 + 2^29 limbs, about 113 GiB of device memory with the twiddle tables).
+// Column-pass twiddle offsets are 32-bit byte offsets below 4N; the reference's
+// own limit is 2^55 (src/data.h:69).
+constexpr int kMaxLogN = 30;
 
 // LDS of the pass kernels.  gfx950 gives a workgroup 160 KiB, static LDS
 // included.  The tuning clamp keeps every tile within the bounds below, and

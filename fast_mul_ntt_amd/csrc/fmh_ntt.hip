@@ -362,7 +362,8 @@ __device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) 
 // Per group the twiddle rows are read as uniform row pointers W + (k hl <<
 // logB) plus one per-lane 32-bit byte offset ((u << logB) + b0 + c) 8, so
 // the loads use a scalar base and no 64-bit address arithmetic per lane
-// (indices stay below 2^28, byte offsets below 2^31).
+// (offsets within a stage stay below its half-span H <= N/2 words, so byte
+// offsets stay below 4N <= 2^32 for transforms of up to 2^30 points).
 __device__ __forceinline__ u64 load_at(const u64* base, u32 byte_off) {
     return *(const FMH_GLOBAL u64*)((const FMH_GLOBAL char*)gmem(base) + byte_off);
 }
@@ -1041,6 +1042,7 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
 // kernels' static LDS (the column kernel has none; the row kernel has the line
 // table).
 static_assert(col_lds_bytes(kMaxColTileLog, 0) <= kLdsPerGroup, "column tile LDS");
+static_assert(kMaxLogN <= 30, "column-pass twiddle byte offsets are 32-bit");
 static_assert(row_lds_bytes(2, kMaxRowTileLog, 0) + sizeof(RowLine) * kMaxRowLines <= kLdsPerGroup,
               "row tile LDS");
 
@@ -1130,7 +1132,7 @@ __device__ __forceinline__ void crt_load(const ProductDesc& D, u64 k, u64& v0, u
         return;
     }
     const u32 m = (u32)D.split_m;
-    const u32 i = (u32)k / m;  // k < 2^28
+    const u32 i = (u32)k / m;  // k < 2^30
     const u64 r = k - (u64)i * m;
     if (i < D.split_n) {
         const FMH_GLOBAL u64* q = gmem(D.res + (u64)i * 3 * N + r);

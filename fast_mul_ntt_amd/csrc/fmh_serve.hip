@@ -132,12 +132,20 @@ struct Request {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr;  // recorded before a group's first launch
+    hipEvent_t done = nullptr;   // after its last
     void* staging = nullptr;  // pinned copy of the group's descriptors
     size_t staging_bytes = 0;
     bool busy = false;
     std::vector<Request*> reqs;
     u64 ws_off = 0, ws_bytes = 0;
+    u32 klass = 0;  // (kind << 8) | log_n of the group
+};
+
+// Device time of the groups of one size class.
+struct ClassTime {
+    u64 groups = 0, requests = 0;
+    double device_ms = 0;  // sum over groups of start -> done on their stream
 };
 
 }  // namespace
@@ -162,6 +170,11 @@ struct fmh_server {
     std::vector<int64_t> latencies;  // ns, completed requests
     Twiddles tw;
     std::string loop_error;
+    // Device activity: each group's interval [start, done] in ms after `ref`
+    // (an event recorded at creation), and its time per size class.
+    hipEvent_t ref = nullptr;
+    std::vector<std::pair<float, float>> busy;
+    std::map<u32, ClassTime> class_time;
     // No group in flight and the oldest request does not fit: only releasing
     // outputs can free memory.  Waits return an error instead of blocking.
     bool stalled = false;
@@ -184,6 +197,15 @@ struct fmh_server {
                     pool.release(r->out_off, r->out_len * sizeof(u64));
                 }
                 completed_unreported.push_back(r->id);
+            }
+            float t0 = 0, t1 = 0;
+            if (q == hipSuccess && hipEventElapsedTime(&t0, ref, s.start) == hipSuccess &&
+                hipEventElapsedTime(&t1, ref, s.done) == hipSuccess) {
+                busy.push_back({t0, t1});
+                ClassTime& c = class_time[s.klass];
+                c.groups++;
+                c.requests += s.reqs.size();
+                c.device_ms += t1 - t0;
             }
             t_last_done = t;
             if (q != hipSuccess) loop_error = std::string("launch group failed: ") + hipGetErrorString(q);
@@ -279,6 +301,7 @@ struct fmh_server {
                 e = hipMemcpyAsync(job.d_desc, s.staging, nd, hipMemcpyHostToDevice, s.stream);
             }
         }
+        if (e == hipSuccess) e = hipEventRecord(s.start, s.stream);
         if (e == hipSuccess) e = job_launch(job, tw, s.stream);
         if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
         const int64_t t = now_ns();
@@ -298,6 +321,7 @@ struct fmh_server {
             return;
         }
         s.busy = true;
+        s.klass = (group[0]->kind << 8) | (u32)group[0]->log_n;
         s.reqs = group;
         s.ws_off = ws_off;
         s.ws_bytes = ws;
@@ -404,15 +428,29 @@ int fmh_server_create(const fmh_server_config* cfg, fmh_server** out) {
     s->slots.resize(s->cfg.streams);
     for (Slot& sl : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&sl.start);
+        if (e == hipSuccess) e = hipEventCreate(&sl.done);
         if (e != hipSuccess) {
             for (Slot& t : s->slots) {
+                if (t.start) (void)hipEventDestroy(t.start);
                 if (t.done) (void)hipEventDestroy(t.done);
                 if (t.stream) (void)hipStreamDestroy(t.stream);
             }
             delete s;
             return fail(std::string("stream setup: ") + hipGetErrorString(e));
         }
+    }
+    hipError_t e = hipEventCreate(&s->ref);
+    if (e == hipSuccess) e = hipEventRecord(s->ref, s->slots[0].stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->ref);
+    if (e != hipSuccess) {
+        for (Slot& t : s->slots) {
+            (void)hipEventDestroy(t.start);
+            (void)hipEventDestroy(t.done);
+            (void)hipStreamDestroy(t.stream);
+        }
+        delete s;
+        return fail(std::string("event setup: ") + hipGetErrorString(e));
     }
     s->loop = std::thread([s] { s->run(); });
     *out = s;
@@ -521,7 +559,41 @@ int fmh_server_stats(fmh_server* s, fmh_server_info* st) {
         st->products_per_s = span > 0 ? (double)s->completed / span : 0.0;
         st->span_s = span;
     }
+    // the union of the groups' device intervals
+    std::vector<std::pair<float, float>> iv = s->busy;
+    std::sort(iv.begin(), iv.end());
+    double busy_ms = 0, end = -1e30;
+    for (const auto& x : iv) {
+        if (x.first > end) {
+            busy_ms += x.second - x.first;
+            end = x.second;
+        } else if (x.second > end) {
+            busy_ms += x.second - end;
+            end = x.second;
+        }
+    }
+    st->device_busy_s = busy_ms * 1e-3;
     return 0;
+}
+
+u64 fmh_server_class_report(fmh_server* s, char* buf, u64 buflen) {
+    if (!s) return 0;
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        char line[160];
+        for (const auto& kv : s->class_time) {
+            snprintf(line, sizeof(line), "%u %u %llu %llu %.3f\n", kv.first >> 8, kv.first & 0xFFu,
+                     (unsigned long long)kv.second.groups, (unsigned long long)kv.second.requests, kv.second.device_ms);
+            out += line;
+        }
+    }
+    if (buf && buflen) {
+        const size_t n = std::min<size_t>(out.size(), (size_t)buflen - 1);
+        memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
+    return (u64)out.size() + 1;
 }
 
 int fmh_server_reset_stats(fmh_server* s) {
@@ -530,6 +602,8 @@ int fmh_server_reset_stats(fmh_server* s) {
     if (s->completed + s->failed != s->submitted) return fail("requests still in flight");
     s->submitted = s->completed = s->failed = s->groups = s->grouped = 0;
     s->latencies.clear();
+    s->busy.clear();
+    s->class_time.clear();
     return 0;
 }
 
@@ -543,11 +617,13 @@ int fmh_server_destroy(fmh_server* s) {
     s->loop.join();
     for (Slot& sl : s->slots) {
         (void)hipStreamSynchronize(sl.stream);
+        (void)hipEventDestroy(sl.start);
         (void)hipEventDestroy(sl.done);
         (void)hipStreamDestroy(sl.stream);
         if (sl.staging) (void)hipHostFree(sl.staging);
     }
     for (auto& kv : s->reqs) delete kv.second;
+    (void)hipEventDestroy(s->ref);
     delete s;
     return 0;
 }

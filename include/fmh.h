@@ -149,6 +149,8 @@ typedef struct {
     double products_per_s;   /* completed / span_s */
     double span_s;           /* first submit -> last completion */
     u64 pool_bytes, pool_peak_bytes;
+    double device_busy_s;    /* time with at least one launch group executing
+                                (HIP events around each group) */
 } fmh_server_info;
 
 typedef struct fmh_server fmh_server;
@@ -171,7 +173,14 @@ int fmh_server_release(fmh_server* server, u64 id);
  * while submitting. */
 int fmh_server_drain(fmh_server* server);
 int fmh_server_stats(fmh_server* server, fmh_server_info* stats);
-/* Zeroes the counters and latencies (no request in flight). */
+/* Device time per size class since the last reset: one line per class,
+ * "kind log_n groups requests device_ms" (kind 0 multiply, 1 square, 2 with a
+ * spectrum, 3 square of a spectrum, 4 an unbalanced multiply split into
+ * pieces of 2^log_n points; device_ms sums each group's execution on
+ * its stream, so groups that overlap count in full).  Writes at most buflen
+ * bytes and returns the bytes needed, terminating NUL included. */
+u64 fmh_server_class_report(fmh_server* server, char* buf, u64 buflen);
+/* Zeroes the counters, latencies and device times (no request in flight). */
 int fmh_server_reset_stats(fmh_server* server);
 /* Drains, stops the scheduler and frees the pool. */
 int fmh_server_destroy(fmh_server* server);

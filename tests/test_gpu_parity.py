@@ -370,21 +370,31 @@ def test_resident_spectrum_shared_by_two_streams(fm, orc):
         s2.close()
 
 
-def test_largest_transform_and_refusal_beyond(fm, orc):
-    """A 2^27-limb square fills the largest transform (2^28 points): checked
-    through residues mod random 61-bit primes.  (The refusal of one
+@pytest.mark.timeout(900)
+def test_largest_transforms_mod_q(fm, orc):
+    """A 2^28 x 2^28-limb multiply (2^29 points) and a 2^29-limb square,
+    which fills the largest transform (2^30 points): checked through residues
+    mod random 61-bit primes and the lowest limb.  (The refusal of one
     coefficient more, like every refusal of invalid pointers or lengths, is
     checked in the CPU emulation only: tests/emu/emu_check.py.)"""
-    import ctypes
-    n = 1 << 27
-    a = orc.splitmix(2718, n)
-    got = fm.abs_sqr64(a)
-    assert got.size == 2 * n
-    assert int(got[0]) == (int(a[0]) * int(a[0])) & 0xFFFFFFFFFFFFFFFF
-    for q in _primes61(2, 77):
-        qa = orc.mod_q(a, q)
-        assert orc.mod_q(got, q) == orc.mulmod_q(qa, qa, q), q
-    assert fm.lib().fmh_max_ntt_len() == 1 << 28
+    from checks import pool
+    assert fm.lib().fmh_max_ntt_len() == 1 << 30
+    qs = _primes61(2, 77)
+    for n1, n2 in [(1 << 28, 1 << 28), (1 << 29, 0)]:
+        print(f"largest: {n1} x {n2 or n1} limbs", flush=True)
+        a = orc.splitmix(2718 + n1, n1)
+        b = orc.splitmix(2719 + n1, n2) if n2 else a
+        got = fm.abs_mul64(a, b) if n2 else fm.abs_sqr64(a)
+        assert got.size == a.size + b.size
+        assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFFFFFFFFFF
+        with pool(6) as ex:
+            ra = [ex.submit(orc.mod_q, a, q) for q in qs]
+            rb = [ex.submit(orc.mod_q, b, q) for q in qs] if n2 else ra
+            rc = [ex.submit(orc.mod_q, got, q) for q in qs]
+            for q, x, y, z in zip(qs, ra, rb, rc):
+                assert z.result() == orc.mulmod_q(x.result(), y.result(), q), (n1, n2, q)
+        del a, b, got
+        assert fm.lib().fmh_trim() == 0  # workspaces of this size are not needed again
 
 
 def _primes61(count, seed):
