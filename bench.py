@@ -40,24 +40,32 @@ SEED_A, SEED_B = 2001, 2002
 HBM_PEAK_GBS = 8000.0
 # Integer VALU: 256 CUs x 64 lanes x 2.4 GHz full-rate lane operations per second.
 VALU_LANE_OPS = 256 * 64 * 2.4e9
-# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against the copy
-# kernel) from rocprofv3 --pmc runs of this workload; profiles/ holds them.
+# HBM bytes per launch (FETCH_SIZE x 2 + WRITE_SIZE, the factors calibrated
+# against the copy kernel) from rocprofv3 --pmc runs of this workload, and the
+# same runs' VALUBusy per kernel (% of one VALU instruction per SIMD every 4
+# cycles); profiles/abs_mul64_1Mx1M_kernels.md holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 203084244.0,
-    "row_mul": 154806686.0,
-    "col_inv": 151434516.0,
-    "crt_local": 70206947.0,
-    "carry_apply": 2386745.0,
+    "col_fwd": 179572267.0,
+    "row_mul": 157091569.0,
+    "col_inv": 160959906.0,
+    "crt_local": 70220418.0,
+    "carry_apply": 2386876.0,
 }
+VALU_BUSY: dict[str, float] = {"col_fwd": 78.4, "row_mul": 76.8, "col_inv": 53.3, "crt_local": 58.1, "carry_apply": 12.5}
+PROFILE = "profiles/abs_mul64_1Mx1M_kernels.md"
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
-# the two column passes).
+# the two column passes): profiles/abs_mul64_16Mx16M_kernels.md.
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3518029302.0,
-    "row_mul": 2419013710.0,
-    "col_inv": 2042241809.0,
-    "crt_local": 1120999807.0,
-    "carry_apply": 38289045.0,
+    "col_fwd": 3353535972.0,
+    "row_mul": 2424239940.0,
+    "col_inv": 2413008485.0,
+    "crt_local": 1121039309.0,
+    "chunk_scan": 269170.0,
+    "carry_apply": 38274278.0,
 }
+VALU_BUSY_16M: dict[str, float] = {"col_fwd": 71.8, "row_mul": 86.1, "col_inv": 67.1, "crt_local": 108.9,
+                                   "chunk_scan": 0.0, "carry_apply": 20.6}
+PROFILE_16M = "profiles/abs_mul64_16Mx16M_kernels.md"
 
 
 def kernel_alg_bytes(n1: int, n2: int, log_n: int, col_passes: int = 1) -> dict[str, float]:
@@ -402,7 +410,7 @@ def run_sizes(args, fm) -> dict:
 
 
 def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str,
-               traffic: dict[str, float]) -> dict:
+               traffic: dict[str, float], valu_busy: dict[str, float], profile: str) -> dict:
     """One device-resident n1 x n2 multiply per step (the headline shape by
     default); per-kernel breakdown, roofline of the dominant kernel, host-ABI
     time and the CPU baselines on the same operands."""
@@ -482,6 +490,8 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
             entry["GBps"] = round(alg[name] / (avg_us * 1e-6) / 1e9, 1)
         if name in traffic:
             entry["traffic_bytes"] = traffic[name]
+        if name in valu_busy:
+            entry["valu_busy_pct"] = valu_busy[name]
         per_kernel[name] = entry
     kernel_sum_us = sum(v["avg_us"] * v["launches_per_step"] for v in per_kernel.values())
     dom_cnt, dom_ms = kt[dom]
@@ -522,7 +532,14 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic.get(dom),
         },
+        # The binding resource is the integer VALU, not HBM (the roofline block
+        # above keeps the HBM form): the dominant kernel's VALUBusy as measured
+        # by rocprofv3 at this geometry, with its source.
+        "roofline_binding": "valu",
         "valu": {
+            "kernel": dom,
+            "busy_frac": round(valu_busy[dom] / 100.0, 3) if dom in valu_busy else None,
+            "busy_source": f"{profile} (rocprofv3 --pmc VALUBusy, same command)",
             "modmuls_per_product": mm,
             "Gmodmul_per_s": round(mm / (ms_per_step * 1e-3) / 1e9, 2),
             "lane_ops_per_modmul_at_peak": round(VALU_LANE_OPS * ms_per_step * 1e-3 / mm, 1),
@@ -604,9 +621,9 @@ def main() -> int:
         print(json.dumps(run_serve(args, fm)))
         return 0
     if args.workload == "mul16m":
-        res = run_single(args, fm, 1 << 24, 1 << 24, 3001, 3002, METRIC_16M, TRAFFIC_16M)
+        res = run_single(args, fm, 1 << 24, 1 << 24, 3001, 3002, METRIC_16M, TRAFFIC_16M, VALU_BUSY_16M, PROFILE_16M)
     else:
-        res = run_single(args, fm, N1, N2, SEED_A, SEED_B, METRIC, TRAFFIC)
+        res = run_single(args, fm, N1, N2, SEED_A, SEED_B, METRIC, TRAFFIC, VALU_BUSY, PROFILE)
     print(json.dumps(res))
     return 0
 
