@@ -9,8 +9,9 @@ keeps the fastest; for classes up to 2^17 it also tunes the geometry of
 batched launches, timing one batch of many such products (the table's
 "b L spec" lines, used when a launch carries several products of a class).
 The search is coordinate descent over the parameters of a geometry spec,
-starting from the defaults (batched classes: from the class's single-product
-entry):
+starting from the class's current entry of the table (with --from-defaults:
+from the built-in defaults; batched classes then start from the class's
+single-product entry):
 
     row_maxl     points per row-pass line          2^8 .. 2^12
     row_tile     points per row-pass tile          2^9 .. 2^13
@@ -19,6 +20,7 @@ entry):
     col_threads  threads per column workgroup      64 .. 1024
     row_threads  threads per row workgroup         256 .. 1024
     row_radix    stages per row-pass LDS round     3 or 4
+    col_dma      forward column tiles by LDS DMA    0 or 1
 
 A candidate replaces the incumbent only if it is at least 3% faster in
 interleaved measurements (candidate and incumbent timed alternately, the
@@ -53,6 +55,7 @@ CANDIDATES = {
     "col_threads": [64, 128, 256, 512, 1024],
     "row_threads": [256, 512, 1024],
     "row_radix": [3, 4],
+    "col_dma": [0, 1],
 }
 DEFAULT_TABLE = os.path.join(HERE, "tuning_gfx950.txt")
 
@@ -139,12 +142,17 @@ class Case:
             d.free()
 
 
-def tune(L: int, stream: fm.Stream, reps: int, rounds: int, count: int = 1) -> tuple[str, float, float]:
+def tune(L: int, stream: fm.Stream, reps: int, rounds: int, count: int = 1,
+         from_table: bool = True) -> tuple[str, float, float]:
     case = Case(L, stream, count)
     try:
-        # batched classes start from the class's single-product geometry
-        fm.set_tuning(L, "", case.batched)
-        base = parse(fm.get_tuning(L, False) if case.batched else fm.get_tuning(L, case.batched))
+        if from_table:
+            # the incumbent is the table's entry: a change must beat it
+            base = parse(fm.get_tuning(L, case.batched))
+        else:
+            # batched classes start from the class's single-product geometry
+            fm.set_tuning(L, "", case.batched)
+            base = parse(fm.get_tuning(L, False) if case.batched else fm.get_tuning(L, case.batched))
         case.check(fmt(base))  # the reference product
         t = case.burst(fmt(base))
         case.per = max(1, int(3e-3 / t))
@@ -182,24 +190,27 @@ def main() -> int:
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--batched-only", action="store_true",
                     help="tune only the batched classes; keep the single-product lines of --out")
+    ap.add_argument("--from-defaults", action="store_true",
+                    help="start each class from the built-in defaults instead of its table entry")
     args = ap.parse_args()
     if fm.lib().fmh_init(0) != 0:
         raise fm.FmhError(fm.lib().fmh_last_error().decode())
     stream = fm.Stream()
     lines = ["# NTT pass geometry per transform length 2^L (written by fast_mul_ntt_amd/autotune.py)",
-             "# L spec      # us tuned / us default, one balanced 2^(L-1) x 2^(L-1) multiply",
-             "# b L spec    # the same for one batched launch of max(8, 2^22 / 2^L) such products"]
+             "# L spec      # us tuned / us start, one balanced 2^(L-1) x 2^(L-1) multiply",
+             "# b L spec    # the same for one batched launch of max(8, 2^22 / 2^L) such products",
+             "# (start: the previous entry of this table, or the defaults with --from-defaults)"]
     if args.batched_only:
         with open(args.out) as f:
             lines += [ln.rstrip("\n") for ln in f if ln[:1].isdigit()]
     else:
         for L in range(args.min_log, args.max_log + 1):
-            spec, us, d_us = tune(L, stream, args.reps, args.rounds)
+            spec, us, d_us = tune(L, stream, args.reps, args.rounds, from_table=not args.from_defaults)
             lines.append(f"{L} {spec}  # {us:.1f} / {d_us:.1f}")
             print(lines[-1], flush=True)
     for L in range(args.min_log, args.max_batch_log + 1):
         count = max(8, (1 << 22) >> L)
-        spec, us, d_us = tune(L, stream, args.reps, args.rounds, count)
+        spec, us, d_us = tune(L, stream, args.reps, args.rounds, count, from_table=not args.from_defaults)
         lines.append(f"b {L} {spec}  # {us:.1f} / {d_us:.1f}")
         print(lines[-1], flush=True)
     stream.close()

@@ -774,8 +774,8 @@ int fmh_get_tuning(int log_n, int batched, char* buf, u64 buflen) {
     if (log_n < 0 || log_n > kMaxLogN || !buf || !buflen) return set_err("bad argument");
     const Tuning t = tuning(log_n, batched != 0);
     snprintf(buf, buflen,
-             "col_logc=%d,col_maxl=%d,row_maxl=%d,row_tile=%d,col_threads=%d,row_threads=%d,row_radix=%d",
-             t.col_logc, t.col_maxl, t.row_maxl, t.row_tile, t.col_threads, t.row_threads, t.row_radix);
+             "col_logc=%d,col_maxl=%d,row_maxl=%d,row_tile=%d,col_threads=%d,row_threads=%d,row_radix=%d,col_dma=%d",
+             t.col_logc, t.col_maxl, t.row_maxl, t.row_tile, t.col_threads, t.row_threads, t.row_radix, t.col_dma);
     return 0;
 }
 

@@ -54,8 +54,8 @@ constexpr int kCrtPerThread = kCarryChunk / kCrtThreads;
 //   * fmh_set_tuning(log_n, spec) at run time;
 //   * $FMH_TUNE (one spec for every size; for sweeps and tests).
 // A spec reads "col_logc=3,col_maxl=10,row_maxl=11,row_tile=12,
-// col_threads=512,row_threads=512,row_radix=4"; missing keys keep their
-// value.
+// col_threads=512,row_threads=512,row_radix=4,col_dma=1"; missing keys keep
+// their value.
 struct Tuning {
     int col_logc = 3;       // column passes: 2^col_logc adjacent lines per tile
     int col_maxl = 10;      // column passes: at most 2^col_maxl points per line
@@ -65,6 +65,7 @@ struct Tuning {
     int col_threads = 512;  // threads per workgroup, column passes
     int row_threads = 512;  // threads per workgroup, row pass
     int row_radix = 4;      // row pass: at most row_radix stages per LDS round (3 or 4)
+    int col_dma = 1;        // forward column passes: fill tiles by LDS DMA where possible (0 or 1)
 };
 // batched: the geometry for a launch that carries several products of the
 // class (table lines "b <log_n> <spec>"); single products use the plain

@@ -117,6 +117,7 @@ static bool parse_tuning(const std::string& spec, Tuning& t) {
         else if (k == "col_threads") t.col_threads = v;
         else if (k == "row_threads") t.row_threads = v;
         else if (k == "row_radix") t.row_radix = v;
+        else if (k == "col_dma") t.col_dma = v;
         else return false;
     }
     return true;
@@ -132,6 +133,7 @@ static Tuning clamp_tuning(Tuning t) {
     t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
     t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
     t.row_radix = t.row_radix == 3 ? 3 : 4;
+    t.col_dma = t.col_dma ? 1 : 0;
     return t;
 }
 
@@ -349,6 +351,33 @@ __device__ __forceinline__ void batched_copy(int n, int nthr, Get get, Put put) 
     }
 }
 
+// LDS DMA (global_load_lds_dwordx4): each lane of a wave moves the 16 bytes
+// at its own global address src into LDS at dst + 16 lane, where dst is the
+// wave's (uniform) LDS address; the data never passes through registers, so
+// a wave can have a whole tile in flight.  lds_dma_wait() waits for this
+// wave's transfers (vmcnt counts them); a barrier then publishes them to the
+// workgroup.
+__device__ __forceinline__ void lds_dma16(const u64* src, u64* dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds((const FMH_GLOBAL void*)src, (__attribute__((address_space(3))) void*)dst, 16,
+                                     0, 0);
+#else
+    memcpy(dst + 2 * (threadIdx.x & 63), src, 16);
+#endif
+}
+__device__ __forceinline__ void lds_dma_wait() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
+}
+__device__ __forceinline__ int wave_uniform(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+
 // ---------------------------------------------------------------- LDS tiles
 //
 // A column tile holds C = 2^logC lines of L = 2^logL words: word (c, j) at
@@ -435,14 +464,16 @@ __device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, 
 }
 
 // Round `first` of a tile reads the transform (ld, fix), round `last` writes
-// it (st); the rounds between go through the tile t in LDS.
-template <int P, int R, bool DIF, class Ld, class Fix, class St>
+// it (st); the rounds between go through the tile t in LDS.  IN_LDS: the
+// tile already holds the transform's words (filled by LDS DMA), and ld reads
+// them from there.
+template <int P, int R, bool DIF, bool IN_LDS, class Ld, class Fix, class St>
 __device__ __forceinline__ void col_round_io(bool first, bool last, u64* __restrict__ t, int logL, int logC, int logB,
                                              u32 b0, const u64* __restrict__ W, int lq, int nthr, Ld ld, Fix fix,
                                              St st) {
     // four data words of the first round in flight per thread: eight (two
     // radix-4 groups) took the kernel from 42 to 94 VGPRs and ran slower
-    constexpr int GI = 4 >> R;
+    constexpr int GI = IN_LDS ? 1 : 4 >> R;
     auto tld = [&](int e) { return t[e]; };
     auto same = [](int, u64 v) { return v; };
     auto tst = [&](int e, u64 v) { t[e] = v; };
@@ -460,7 +491,7 @@ __device__ __forceinline__ void col_round_io(bool first, bool last, u64* __restr
 // The stages of a column tile (logL >= 1): DIF as radix-2^2 rounds over
 // hl = L/2 .. 1 with one radix-2 stage last when logL is odd; DIT as the
 // radix-2 stage first when logL is odd, then radix-2^2 rounds up to hl = L/2.
-template <int P, bool DIF, class Ld, class Fix, class St>
+template <int P, bool DIF, bool IN_LDS, class Ld, class Fix, class St>
 __device__ __forceinline__ void tile_rounds(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
                                             const u64* __restrict__ W, int nthr, Ld ld, Fix fix, St st) {
     const int nr = (logL >> 1) + (logL & 1);
@@ -469,10 +500,10 @@ __device__ __forceinline__ void tile_rounds(u64* __restrict__ t, int logL, int l
         const bool first = i == 0, last = i == nr - 1;
         const bool r1 = odd && (DIF ? last : first);
         if (r1) {
-            col_round_io<P, 1, DIF>(first, last, t, logL, logC, logB, b0, W, 0, nthr, ld, fix, st);
+            col_round_io<P, 1, DIF, IN_LDS>(first, last, t, logL, logC, logB, b0, W, 0, nthr, ld, fix, st);
         } else {
             const int lq = DIF ? logL - 2 * (i + 1) : (logL & 1) + 2 * (i - (odd ? 1 : 0));
-            col_round_io<P, 2, DIF>(first, last, t, logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
+            col_round_io<P, 2, DIF, IN_LDS>(first, last, t, logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
         }
     }
 }
@@ -509,7 +540,22 @@ __device__ __forceinline__ u32 xcd_slot(u32 x, u32 n) {
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
-template <int P, bool FWD>
+// Fills the column tile (n words, t[e] = word at(e) of src) by LDS DMA.
+// Tile words 2i and 2i + 1 lie in one tile row (C >= 2 adjacent words), so
+// each lane moves one such pair, and each wave instruction fills 128 words
+// (1 KiB) of the tile.  Pairs for which skip(e) holds are not loaded.
+template <class At, class Skip>
+__device__ __forceinline__ void col_dma_fill(u64* t, int n, int nthr, const u64* src, At at, Skip skip) {
+    const int lane2 = (int)(threadIdx.x & 63) << 1;
+    for (int k = wave_uniform((int)threadIdx.x >> 6) << 7; k < n; k += nthr << 1) {
+        const int e = k + lane2;
+        if (e < n && !skip(e)) lds_dma16(src + at(e), t + k);
+    }
+    lds_dma_wait();
+    __syncthreads();
+}
+
+template <int P, bool FWD, bool DMA>
 __device__ void col_body(const ColArgs& A, u32 tile, int op) {
     extern __shared__ u64 lds[];
     const u32 prod = blockIdx.z;
@@ -530,10 +576,29 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
     // tile word e = j C + c is word base + j B + c of the transform
     auto at = [&](int e) { return base + ((u64)(e >> logC) << logB) + (u64)(e & (C - 1)); };
     auto put = [&](int e, u64 v) { res[at(e)] = v; };
-    if (!A.from_limbs) {
+    auto same = [](int, u64 v) { return v; };
+    if (DMA) {
+        // the tile is filled by DMA, then every round runs from LDS
+        auto tld = [&](int e) { return lds[e]; };
+        if (!A.from_limbs) {
+            col_dma_fill(lds, n, nthr, (const u64*)res, at, [](int) { return false; });
+            tile_rounds<P, FWD, true>(lds, logL, logC, logB, b0, A.tw[P], nthr, tld, same, put);
+        } else {
+            // Pairs reaching past the operand's last word are not loaded: the
+            // words past it are zero, and an odd length's last word (the
+            // first of its pair) is read directly.  The host sends only
+            // operands whose limbs are 16-byte aligned here.
+            col_dma_fill(lds, n, nthr, op ? D.b : D.a, at, [&](int e) { return at(e) + 1 >= len; });
+            auto val = [&](int e, u64 v) {
+                const u64 g = at(e);
+                if (g >= len) return 0ull;
+                return reduce_word<P>((g | 1) >= len ? limbs[g] : v);
+            };
+            tile_rounds<P, FWD, true>(lds, logL, logC, logB, b0, A.tw[P], nthr, tld, val, put);
+        }
+    } else if (!A.from_limbs) {
         auto get = [&](int e) { return res[at(e)]; };
-        auto same = [](int, u64 v) { return v; };
-        tile_rounds<P, FWD>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, same, put);
+        tile_rounds<P, FWD, false>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, same, put);
     } else {
         // an empty operand reads the first twiddle and drops it
         const FMH_GLOBAL u64* __restrict__ src = len ? limbs : gmem(A.tw[P]);
@@ -543,7 +608,7 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
             return src[g < last ? g : last];
         };
         auto val = [&](int e, u64 v) { return at(e) < len ? reduce_word<P>(v) : 0ull; };
-        tile_rounds<P, FWD>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, val, put);
+        tile_rounds<P, FWD, false>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, val, put);
     }
 }
 
@@ -552,7 +617,11 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
 // the same limbs (first forward pass), and neighbouring tiles share 128-byte
 // lines of data and twiddles (a tile row is 2^logC words); all of them run
 // on one XCD at about the same time, so the shared reads hit its L2.
-template <bool FWD>
+// DMA: the tile is filled by LDS DMA (residues, or limbs that are 16-byte
+// aligned; tiles of at least 2 adjacent lines).  Forward passes only: at
+// 2^25 points it took the forward passes from 1050 and 918 us to 1012 and
+// 859 us, and the inverse passes from 451 and 542 us to 445 and 559 us.
+template <bool FWD, bool DMA>
 __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
     const u32 per_tile = 3u * (u32)A.nops;
@@ -561,9 +630,9 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 prime = rem / (u32)A.nops;
     const int op = (int)(rem - prime * (u32)A.nops);
     switch (prime) {
-        case 0: col_body<0, FWD>(A, tile, op); break;
-        case 1: col_body<1, FWD>(A, tile, op); break;
-        default: col_body<2, FWD>(A, tile, op); break;
+        case 0: col_body<0, FWD, DMA>(A, tile, op); break;
+        case 1: col_body<1, FWD, DMA>(A, tile, op); break;
+        default: col_body<2, FWD, DMA>(A, tile, op); break;
     }
 }
 
@@ -1548,7 +1617,8 @@ static const CrtConsts& crt_table(int log_n) {
 // 160 KiB per workgroup, static LDS included; kernels default to 64 KiB
 // unless the limit is raised.  Launches that would exceed it are refused
 // with an error instead of being sent.
-static size_t g_max_dyn[4] = {0, 0, 0, 0};  // k_col<true>, k_col<false>, k_row<4>, k_row<3>
+// k_col<true, false>, k_col<false, false>, k_row<4>, k_row<3>, k_col<true, true>
+static size_t g_max_dyn[5] = {0, 0, 0, 0, 0};
 
 static size_t raise_lds_limit(const void* f) {
     constexpr size_t kDefault = 64 * 1024;
@@ -1566,10 +1636,11 @@ void prepare_kernels() {
     (void)crt_table(0);
     static std::once_flag once;
     std::call_once(once, [] {
-        g_max_dyn[0] = raise_lds_limit((const void*)k_col<true>);
-        g_max_dyn[1] = raise_lds_limit((const void*)k_col<false>);
+        g_max_dyn[0] = raise_lds_limit((const void*)k_col<true, false>);
+        g_max_dyn[1] = raise_lds_limit((const void*)k_col<false, false>);
         g_max_dyn[2] = raise_lds_limit((const void*)k_row<4>);
         g_max_dyn[3] = raise_lds_limit((const void*)k_row<3>);
+        g_max_dyn[4] = raise_lds_limit((const void*)k_col<true, true>);
     });
 }
 
@@ -1797,6 +1868,21 @@ hipError_t job_upload(const Job& j, hipStream_t s) {
     return hipStreamSynchronize(s);  // the host copy goes out of scope
 }
 
+// Whether a forward column pass over the products d[0, count) may fill its
+// tiles by LDS DMA: tiles of at least two adjacent lines, and for the first
+// pass every operand it reads (nops of each product) 16-byte aligned.
+static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const ProductDesc* d, size_t count, int nops) {
+    if (!tu.col_dma || logC < 1) return false;
+    if (!from_limbs) return true;
+    for (size_t i = 0; i < count; i++)
+        for (int op = 0; op < nops; op++) {
+            const u64 len = op ? d[i].nb : d[i].na;
+            const uintptr_t p = (uintptr_t)(op ? d[i].b : d[i].a);
+            if (len && (p & 15)) return false;
+        }
+    return true;
+}
+
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
@@ -1823,9 +1909,13 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
-        if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3 * nops), 1, count), dim3(tu.col_threads), lds,
-                   s, C);
+        const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &j.items[first], count, nops);
+        if (lds > g_max_dyn[dma ? 4 : 0]) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(tiles * 3 * nops), 1, count);
+        if (dma)
+            FMH_LAUNCH("col_fwd", (k_col<true, true>), grid, dim3(tu.col_threads), lds, s, C);
+        else
+            FMH_LAUNCH("col_fwd", (k_col<true, false>), grid, dim3(tu.col_threads), lds, s, C);
     }
     {
         RowArgs R{};
@@ -1873,7 +1963,8 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
         if (lds > g_max_dyn[1]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_inv", k_col<false>, dim3((unsigned)(tiles * 3), 1, count), dim3(tu.col_threads), lds, s, C);
+        FMH_LAUNCH("col_inv", (k_col<false, false>), dim3((unsigned)(tiles * 3), 1, count), dim3(tu.col_threads), lds,
+                   s, C);
     }
     return hipSuccess;
 }
@@ -1967,8 +2058,14 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
-        if (lds > g_max_dyn[0]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_fwd", k_col<true>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds, s, C);
+        const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &d, 1, 1);
+        if (lds > g_max_dyn[dma ? 4 : 0]) return hipErrorInvalidValue;
+        if (dma)
+            FMH_LAUNCH("col_fwd", (k_col<true, true>), dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds,
+                       s, C);
+        else
+            FMH_LAUNCH("col_fwd", (k_col<true, false>), dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds,
+                       s, C);
     }
     RowArgs R{};
     R.one = d;

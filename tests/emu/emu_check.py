@@ -118,6 +118,25 @@ check(not out8.any(), "a x (empty) is zero")
 check(L.fmh_mul_device(a8.ctypes.data, 8, None, 0, None, None) != 0, "null output of an empty product")
 check(L.fmh_sqr_device(buf.ctypes.data, 8, None, None) != 0, "null square output")
 check(b"NULL" in L.fmh_last_error(), "null square output message")
+# operands at 16-byte aligned and at odd word offsets, odd and even lengths,
+# products with a column pass: the first column pass fills its tiles by LDS
+# DMA from aligned limbs (an odd length's last word read on its own) and
+# word by word from the others.  Offset None: an array of its own, whose
+# last word ends its allocation (a read past it would stop AddressSanitizer).
+# The longer products give the column passes rounds of 3 stages (radix 8).
+pool_words = orc.splitmix(4321, 72000)
+for off1, n1, off2, n2 in [(0, 4097, 0, 4000), (1, 4097, 0, 3), (0, 4096, 5, 4001), (3, 2999, 7, 2049),
+                           (0, 9001, 0, 9000), (1, 9001, 2, 1), (None, 4097, None, 2049), (None, 6001, None, 1),
+                           (0, 20001, 1, 20000), (0, 40001, 1, 30000)]:
+    a = orc.splitmix(n1, n1) if off1 is None else pool_words[off1:off1 + n1]
+    b = orc.splitmix(n2 + 1, n2) if off2 is None else pool_words[off2:off2 + n2]
+    out = np.full(n1 + n2, 3, np.uint64)
+    check(L.fmh_mul_device(a.ctypes.data, n1, b.ctypes.data, n2, out.ctypes.data, None) == 0, ("offset rc", off1))
+    check(np.array_equal(out, orc.mul(a.copy(), b.copy())), ("offset mul", off1, n1, off2, n2))
+    out = np.full(2 * n1, 3, np.uint64)
+    check(L.fmh_sqr_device(a.ctypes.data, n1, out.ctypes.data, None) == 0, ("offset sqr rc", off1))
+    check(np.array_equal(out, orc.sqr(a.copy())), ("offset sqr", off1, n1))
+print("operand offsets ok", flush=True)
 # lengths beyond the largest transform are refused before any memory is read
 # (the buffer is far smaller than the lengths)
 L.fmh_max_ntt_len.restype = u

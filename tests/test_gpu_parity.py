@@ -88,6 +88,36 @@ def test_every_tuning_candidate_gives_the_same_product(fm, orc):
             fm.set_tuning(lg, t)
 
 
+@pytest.mark.parametrize("col_dma", [0, 1])
+def test_operands_at_any_word_offset(fm, orc, col_dma):
+    """Device operands at 16-byte aligned and odd word offsets of one buffer,
+    odd and even lengths, with the forward column tiles filled by LDS DMA
+    (aligned operands; the last word of an odd length read on its own) or
+    word by word: products and squares equal the oracle's."""
+    src_host = orc.splitmix(4711, 140000)
+    src = fm.DeviceBuffer.from_host(src_host)
+    saved = {lg: fm.get_tuning(lg) for lg in (13, 14, 16, 17)}
+    try:
+        for lg in saved:
+            fm.set_tuning(lg, f"col_dma={col_dma}")
+        for off1, n1, off2, n2 in [(0, 40001, 1, 30000), (1, 40001, 0, 3), (2, 65536, 5, 65535), (3, 6001, 4, 2000)]:
+            a, b = src_host[off1:off1 + n1], src_host[off2:off2 + n2]
+            out = fm.DeviceBuffer(n1 + n2)
+            sq = fm.DeviceBuffer(2 * n1)
+            try:
+                fm.mul_device(src.view(off1, n1), n1, src.view(off2, n2), n2, out)
+                fm.sqr_device(src.view(off1, n1), n1, sq)
+                assert np.array_equal(out.download(), orc.mul(a.copy(), b.copy())), (off1, n1, off2, n2)
+                assert np.array_equal(sq.download(), orc.sqr(a.copy())), (off1, n1)
+            finally:
+                out.free()
+                sq.free()
+    finally:
+        for lg, t in saved.items():
+            fm.set_tuning(lg, t)
+        src.free()
+
+
 def test_every_batched_tuning_candidate_gives_the_same_products(fm, orc):
     """The same geometries applied to batched launches: a batch of 5
     products of the 2^14 class."""
