@@ -3,14 +3,15 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include <atomic>
-#include <barrier>
 #include <chrono>
 #include <functional>
 #include <memory>
 #include <mutex>
-#include <thread>
 #include <vector>
+
+#include <sanitizer/common_interface_defs.h>
+#include <sys/mman.h>
+#include <ucontext.h>
 
 #include "hip/hip_runtime.h"
 
@@ -19,28 +20,55 @@ thread_local dim3 blockIdx;
 dim3 blockDim;
 dim3 gridDim;
 
-static std::barrier<>* g_bar = nullptr;
+// The threads of a workgroup are fibers (ucontext) on the launching host
+// thread.  A fiber runs until it reaches a barrier or returns; once every
+// fiber of the workgroup has done one or the other, the waiting ones resume.
+// So __syncthreads has the GPU's meaning without host-thread barriers, and
+// AddressSanitizer follows the stack switches through its fiber interface.
+struct Fiber {
+    ucontext_t ctx;
+    char* stack = nullptr;
+    void* fake_stack = nullptr;
+    bool done = false;
+};
+static constexpr size_t kFiberStack = 256 << 10;
+static std::mutex g_launch_mu;  // one grid at a time (the globals below)
+static std::vector<Fiber> g_fibers;
+static ucontext_t g_sched;
+static const void* g_sched_bottom = nullptr;
+static size_t g_sched_size = 0;
+static unsigned g_cur = 0;
+static const std::function<void()>* g_body = nullptr;
 static unsigned g_slots[1024];
 static void* g_dyn = nullptr;
-static size_t g_dyn_bytes = 0;
 
 void* emu_dyn_lds() { return g_dyn; }
 
-static std::atomic<bool> g_used_barrier{false};
-
-void emu_syncthreads() {
-    g_used_barrier = true;
-    g_bar->arrive_and_wait();
+// back to the scheduler: at a barrier (finishing = false) or at the end
+static void to_scheduler(bool finishing) {
+    Fiber& f = g_fibers[g_cur];
+    __sanitizer_start_switch_fiber(finishing ? nullptr : &f.fake_stack, g_sched_bottom, g_sched_size);
+    swapcontext(&f.ctx, &g_sched);
+    __sanitizer_finish_switch_fiber(f.fake_stack, nullptr, nullptr);
 }
 
+static void fiber_main() {
+    __sanitizer_finish_switch_fiber(nullptr, &g_sched_bottom, &g_sched_size);
+    (*g_body)();
+    g_fibers[g_cur].done = true;
+    to_scheduler(true);
+    abort();  // a finished fiber is never resumed
+}
+
+void emu_syncthreads() { to_scheduler(false); }
+
 unsigned emu_shfl_up_u32(unsigned v, unsigned delta, int width) {
-    g_used_barrier = true;
     const unsigned tid = threadIdx.x;
     g_slots[tid] = v;
-    g_bar->arrive_and_wait();
+    emu_syncthreads();
     const unsigned lane = tid % (unsigned)width;
     const unsigned r = lane >= delta ? g_slots[tid - delta] : v;
-    g_bar->arrive_and_wait();
+    emu_syncthreads();
     return r;
 }
 
@@ -55,82 +83,65 @@ struct emu_event {
     std::chrono::steady_clock::time_point t;
 };
 
-// A team of block.x host threads runs every workgroup of one launch in turn.
-// A kernel whose first workgroup never synchronised is barrier-free: its other
-// workgroups run their threads one after the other on the calling thread.
+// Runs the workgroups one after the other, each as block.x fibers.
 static void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, size_t lds) {
     if (block.x == 0 || block.x > 1024 || block.y != 1 || block.z != 1) {
         fprintf(stderr, "emu: bad block shape %u %u %u\n", block.x, block.y, block.z);
         abort();
     }
+    std::lock_guard<std::mutex> lock(g_launch_mu);
     gridDim = grid;
     blockDim = block;
+    g_body = &body;
+    if (g_fibers.size() < block.x) {
+        const size_t old = g_fibers.size();
+        g_fibers.resize(block.x);
+        for (size_t t = old; t < block.x; t++) {
+            void* st = mmap(nullptr, kFiberStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+            if (st == MAP_FAILED) {
+                fprintf(stderr, "emu: fiber stack\n");
+                abort();
+            }
+            g_fibers[t].stack = (char*)st;
+        }
+    }
+    const dim3 saved_t = threadIdx, saved_b = blockIdx;
     const unsigned long long nblocks = (unsigned long long)grid.x * grid.y * grid.z;
-    auto block_of = [&](unsigned long long b) {
-        return dim3((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
-                    (unsigned)(b / ((unsigned long long)grid.x * grid.y)));
-    };
-    auto new_lds = [&] {
+    for (unsigned long long b = 0; b < nblocks; b++) {
         // fresh dynamic LDS of exactly the requested size, filled with a pattern
-        g_dyn_bytes = lds;
         g_dyn = lds ? malloc(lds) : nullptr;
         if (lds) memset(g_dyn, 0xA5, lds);
-    };
-    std::barrier<> bar(block.x);
-    g_bar = &bar;
-    g_used_barrier = false;
-    // first workgroup with real threads
-    new_lds();
-    {
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < block.x; t++)
-            th.emplace_back([&, t] {
-                threadIdx = dim3(t, 0, 0);
-                blockIdx = block_of(0);
-                body();
-            });
-        for (auto& h : th) h.join();
-    }
-    free(g_dyn);
-    g_dyn = nullptr;
-    if (nblocks == 1) return;
-    if (!g_used_barrier) {
-        const dim3 saved_t = threadIdx, saved_b = blockIdx;
-        for (unsigned long long b = 1; b < nblocks; b++) {
-            new_lds();
-            blockIdx = block_of(b);
-            for (unsigned t = 0; t < block.x; t++) {
-                threadIdx = dim3(t, 0, 0);
-                body();
-            }
-            free(g_dyn);
-            g_dyn = nullptr;
+        blockIdx = dim3((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
+                        (unsigned)(b / ((unsigned long long)grid.x * grid.y)));
+        for (unsigned t = 0; t < block.x; t++) {
+            Fiber& f = g_fibers[t];
+            getcontext(&f.ctx);
+            f.ctx.uc_stack.ss_sp = f.stack;
+            f.ctx.uc_stack.ss_size = kFiberStack;
+            f.ctx.uc_link = nullptr;
+            makecontext(&f.ctx, fiber_main, 0);
+            f.done = false;
+            f.fake_stack = nullptr;
         }
-        threadIdx = saved_t;
-        blockIdx = saved_b;
-        return;
-    }
-    std::barrier<> gate(block.x + 1);  // block start / end, with the launching thread
-    std::vector<std::thread> team;
-    team.reserve(block.x);
-    for (unsigned t = 0; t < block.x; t++)
-        team.emplace_back([&, t] {
-            threadIdx = dim3(t, 0, 0);
-            for (unsigned long long b = 1; b < nblocks; b++) {
-                gate.arrive_and_wait();  // block b starts
-                blockIdx = block_of(b);
-                body();
-                gate.arrive_and_wait();  // block b done
+        unsigned live = block.x;
+        while (live) {
+            for (unsigned t = 0; t < block.x; t++) {
+                Fiber& f = g_fibers[t];
+                if (f.done) continue;
+                g_cur = t;
+                threadIdx = dim3(t, 0, 0);
+                void* fake = nullptr;
+                __sanitizer_start_switch_fiber(&fake, f.stack, kFiberStack);
+                swapcontext(&g_sched, &f.ctx);
+                __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
+                if (f.done) live--;
             }
-        });
-    for (unsigned long long b = 1; b < nblocks; b++) {
-        new_lds();
-        gate.arrive_and_wait();
-        gate.arrive_and_wait();
+        }
         free(g_dyn);
         g_dyn = nullptr;
     }
-    for (auto& h : team) h.join();
+    threadIdx = saved_t;
+    blockIdx = saved_b;
 }
 
 void emu_launch(std::function<void()> body, dim3 grid, dim3 block, size_t lds, hipStream_t s) {

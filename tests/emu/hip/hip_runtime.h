@@ -4,8 +4,9 @@
 // against this header, after one textual substitution: the dynamic LDS array
 // ("extern __shared__ u64 lds[]") becomes a pointer to a host buffer of
 // exactly the launch's dynamic LDS size, so its bounds are checked.  Every
-// GPU thread of a workgroup is a host thread; workgroups run one after the
-// other; __syncthreads is a barrier across the workgroup's threads; device
+// GPU thread of a workgroup is a fiber on the launching host thread;
+// workgroups run one after the other; __syncthreads suspends a fiber until
+// every thread of its workgroup has reached a barrier or returned; device
 // memory is host memory, so AddressSanitizer checks every kernel access.
 // Streams run eagerly; graph capture records launches for replay.
 #pragma once
