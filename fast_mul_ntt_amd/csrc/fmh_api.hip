@@ -110,7 +110,10 @@ CopyPool& copy_pool() {
 #define FMH_PIN_CHUNK (4u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
 #endif
 constexpr size_t kPinChunk = FMH_PIN_CHUNK;
-constexpr int kRing = 3;  // 4 MiB DMA chunks run at 51 GB/s, 2 MiB ones at 45 GB/s
+#ifndef FMH_PIN_RING
+#define FMH_PIN_RING 3
+#endif
+constexpr int kRing = FMH_PIN_RING;  // 4 MiB DMA chunks run at 51 GB/s, 2 MiB ones at 45 GB/s
 
 // Device scratch of fmh_mul_device_noalloc: the workspace (residues of both
 // operands for the three primes, per-limb carry maps, per-chunk maps and
@@ -289,6 +292,41 @@ struct Segment {
     size_t bytes;
 };
 
+// Chunks of a staged transfer.  Neither end of a transfer overlaps
+// anything: the first H2D chunk's host copy runs before any DMA, the last D2H
+// chunk's host copy after the last one.  So an upload starts with a chunk of
+// at most kPinHead bytes and a download ends with one; the rest of a
+// transfer is cut into equal chunks of at most kPinChunk bytes (fewer DMAs:
+// each costs a few microseconds of its own).
+#ifndef FMH_PIN_HEAD
+#define FMH_PIN_HEAD (FMH_PIN_CHUNK / 4)
+#endif
+constexpr size_t kPinHead = FMH_PIN_HEAD;
+
+// (offset, bytes) of equal chunks of at most kPinChunk covering [off, off + n).
+void equal_chunks(size_t off, size_t n, std::vector<std::pair<size_t, size_t>>& c) {
+    if (!n) return;
+    const size_t k = (n + kPinChunk - 1) / kPinChunk;
+    const size_t per = ((n + k - 1) / k + 63) & ~(size_t)63;
+    for (size_t o = 0; o < n; o += per) c.push_back({off + o, std::min(per, n - o)});
+}
+
+std::vector<std::pair<size_t, size_t>> h2d_chunks(size_t bytes, bool first) {
+    std::vector<std::pair<size_t, size_t>> c;
+    const size_t head = first ? std::min(bytes, kPinHead) : 0;
+    if (head) c.push_back({0, head});
+    equal_chunks(head, bytes - head, c);
+    return c;
+}
+
+std::vector<std::pair<size_t, size_t>> d2h_chunks(size_t bytes) {
+    std::vector<std::pair<size_t, size_t>> c;
+    const size_t tail = std::min(bytes, kPinHead);
+    equal_chunks(0, bytes - tail, c);
+    if (tail) c.push_back({bytes - tail, tail});
+    return c;
+}
+
 // Pageable host -> device through the pinned ring (a pageable hipMemcpy
 // costs hundreds of microseconds even for small buffers), every segment in
 // one pipeline: the host threads fill buffer k % kRing while the chunks
@@ -297,9 +335,9 @@ int h2d_staged(HostSlot& h, const Segment* seg, int nseg) {
     if (ensure_pinned(h)) return -1;
     size_t k = 0;
     for (int g = 0; g < nseg; g++)
-        for (size_t off = 0; off < seg[g].bytes; off += kPinChunk, k++) {
-            const size_t n = std::min(kPinChunk, seg[g].bytes - off);
-            const int i = (int)(k % kRing);
+        for (const auto& ch : h2d_chunks(seg[g].bytes, k == 0)) {
+            const size_t off = ch.first, n = ch.second;
+            const int i = (int)(k++ % kRing);
             hipError_t e = hipEventSynchronize(h.pin_ev[i]);  // the DMA that last read buffer i
             if (e != hipSuccess) return set_err("hipEventSynchronize", e);
             copy_pool().copy(h.pin[i], (const char*)seg[g].host + off, n);
@@ -316,10 +354,11 @@ int h2d_staged(HostSlot& h, const Segment* seg, int nseg) {
 // byte.
 int d2h_staged(HostSlot& h, void* dst, const void* src, size_t bytes) {
     if (ensure_pinned(h)) return -1;
-    const size_t chunks = (bytes + kPinChunk - 1) / kPinChunk;
+    const std::vector<std::pair<size_t, size_t>> ch = d2h_chunks(bytes);
+    const size_t chunks = ch.size();
     auto issue = [&](size_t k) {
-        const size_t off = k * kPinChunk, n = std::min(kPinChunk, bytes - off);
-        hipError_t r = hipMemcpyAsync(h.pin[k % kRing], (const char*)src + off, n, hipMemcpyDeviceToHost, h.stream);
+        hipError_t r = hipMemcpyAsync(h.pin[k % kRing], (const char*)src + ch[k].first, ch[k].second,
+                                      hipMemcpyDeviceToHost, h.stream);
         if (r == hipSuccess) r = hipEventRecord(h.pin_ev[k % kRing], h.stream);
         return r;
     };
@@ -328,8 +367,7 @@ int d2h_staged(HostSlot& h, void* dst, const void* src, size_t bytes) {
     for (size_t k = 0; e == hipSuccess && k < chunks; k++) {
         e = hipEventSynchronize(h.pin_ev[k % kRing]);
         if (e != hipSuccess) break;
-        const size_t off = k * kPinChunk;
-        copy_pool().copy((char*)dst + off, h.pin[k % kRing], std::min(kPinChunk, bytes - off));
+        copy_pool().copy((char*)dst + ch[k].first, h.pin[k % kRing], ch[k].second);
         if (k + kRing < chunks) e = issue(k + kRing);
     }
     return e == hipSuccess ? 0 : set_err("staged D2H", e);

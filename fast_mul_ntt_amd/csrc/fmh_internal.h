@@ -13,7 +13,7 @@
 namespace fmh {
 
 // Largest transform length handled (2^30 points per prime: products of up to
-// 2^29 + 2^29 limbs, about 113 GiB of device memory with the twiddle tables).
+// 2^29 + 2^29 limbs, about 58 GiB of device memory with the twiddle tables).
 // Column-pass twiddle offsets are 32-bit byte offsets below 4N; the reference's
 // own limit is 2^55 (src/data.h:69).
 constexpr int kMaxLogN = 30;
@@ -96,13 +96,30 @@ struct CrtConsts {
 };
 CrtConsts crt_consts(int log_n);
 
-// Twiddle tables: tw[prime] (forward) and itw[prime] (inverse), each
-// 2^log_t words, entry [h + t] = w_{2h}^{+-t} * 2^64 mod p for 1 <= h < 2^log_t.
+// Twiddle tables covering transforms of up to 2^log_t points, per prime:
+//   fwd / inv   2^log_full words (log_full = min(log_t, F)), entry [h + t] =
+//               w_{2h}^{+-t} * 2^64 mod p for 1 <= h < 2^log_full;
+//   fwd_x/inv_x for the longer half-spans h = 2^e, F <= e < log_t (transforms
+//               above 2^F points): 2^(log_t - F + 1) words, entry [K + t] =
+//               w_{2h}^{+-t} * 2^64 mod p for t < K = h / 2^(F-1).
+// A twiddle of such a row is the product of two entries: with t = t_hi K +
+// t_lo, w_{2h}^t = w_{2^F}^{t_hi} w_{2h}^{t_lo} = fwd[2^(F-1) + t_hi] *
+// fwd_x[K + t_lo].  So the memory held stays at 48 * 2^F bytes however long
+// the transform (1.5 GiB with F = 25, instead of 48 GiB at 2^30 points),
+// and only column stages of half-span >= 2^F pay the extra product.  F is
+// kTwFullLog, or $FMH_TW_FULL_LOG (at least kMaxRowTileLog: row-pass
+// twiddles always come from the full table).
+constexpr int kTwFullLog = 25;
 struct Twiddles {
     int log_t = -1;
+    int log_full = -1;
     u64* fwd[3] = {nullptr, nullptr, nullptr};
     u64* inv[3] = {nullptr, nullptr, nullptr};
+    u64* fwd_x[3] = {nullptr, nullptr, nullptr};
+    u64* inv_x[3] = {nullptr, nullptr, nullptr};
 };
+// F of the tables this process builds.
+int tw_full_log();
 
 hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s);
 

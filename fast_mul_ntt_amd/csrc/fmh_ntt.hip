@@ -404,17 +404,34 @@ __device__ __forceinline__ u64 load_at(const u64* base, u32 byte_off) {
 // transform's words in global memory for the first and the last round of a
 // tile.  G groups of a thread issue their loads together (an index past the
 // end is clamped and its result dropped).
-template <int P, int R, bool DIF, int G, class Ld, class Fix, class St>
-__device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, const u64* __restrict__ W, int lq,
-                                          int nthr, Ld ld, Fix fix, St st) {
+// Twiddle t of the row of half-span 2^lh >= 2^F (lk = lh - F + 1): the
+// product of W[2^(F-1) + (t >> lk)] and X[2^lk + (t mod 2^lk)] (see Twiddles
+// in fmh_internal.h), canonical like the table entries.
+template <int P>
+__device__ __forceinline__ u64 tw_two(const u64* __restrict__ W, const u64* __restrict__ X, int F, int lk, u32 t) {
+    const u64 hi = gmem(W)[(1u << (F - 1)) + (t >> lk)];
+    const u64 lo = gmem(X)[(1u << lk) + (t & ((1u << lk) - 1))];
+    return canon<P>(redc<P>(hi, lo));
+}
+
+// BIG: the launch has rows of half-span 2^F or more (transforms above 2^F
+// points); the rounds that reach them form those twiddles with tw_two.
+template <int P, int R, bool DIF, int G, bool BIG, class Ld, class Fix, class St>
+__device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, const u64* __restrict__ W,
+                                          const u64* __restrict__ X, int F, int lq, int nthr, Ld ld, Fix fix,
+                                          St st) {
     constexpr int K = 1 << R;
     const int C = 1 << logC;
     const int hl = 1 << lq;
     const int nbf = 1 << (logL - R + logC);
     const int sp = hl << logC;  // tile distance of words q and q + 1 of a group
-    const u64* __restrict__ Wa = W + ((u64)(2 * hl) << logB);
-    const u64* __restrict__ Wb = W + ((u64)(3 * hl) << logB);
-    const u64* __restrict__ Wc = W + ((u64)hl << logB);
+    // global half-spans 2^lhc (the round's last stage) and 2^(lhc + 1)
+    const int lhc = lq + logB;
+    const bool two_c = BIG && lhc >= F;
+    const bool two_a = BIG && R == 2 && lhc + 1 >= F;
+    const u64* __restrict__ Wa = two_a ? W : W + ((u64)(2 * hl) << logB);
+    const u64* __restrict__ Wb = two_a ? W : W + ((u64)(3 * hl) << logB);
+    const u64* __restrict__ Wc = two_c ? W : W + ((u64)hl << logB);
     for (int beta0 = threadIdx.x; beta0 < nbf; beta0 += G * nthr) {
         u64 x[G][K], w[G][R == 2 ? 3 : 1];
         int i0[G];
@@ -425,11 +442,20 @@ __device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, 
             const int r = beta >> logC;
             const int u = r & (hl - 1);
             const int j0 = ((r - u) << R) + u;  // (r / hl) 2^R hl + u
-            const u32 off = (((u32)u << logB) + b0 + (u32)c) << 3;
-            w[k][0] = load_at(Wc, off);
+            const u32 t = ((u32)u << logB) + b0 + (u32)c;
+            const u32 off = t << 3;
+            if (two_c)
+                w[k][0] = tw_two<P>(W, X, F, lhc - F + 1, t);
+            else
+                w[k][0] = load_at(Wc, off);
             if (R == 2) {
-                w[k][R == 2 ? 1 : 0] = load_at(Wa, off);
-                w[k][R == 2 ? 2 : 0] = load_at(Wb, off);
+                if (two_a) {
+                    w[k][R == 2 ? 1 : 0] = tw_two<P>(W, X, F, lhc - F + 2, t);
+                    w[k][R == 2 ? 2 : 0] = tw_two<P>(W, X, F, lhc - F + 2, t + ((u32)hl << logB));
+                } else {
+                    w[k][R == 2 ? 1 : 0] = load_at(Wa, off);
+                    w[k][R == 2 ? 2 : 0] = load_at(Wb, off);
+                }
             }
             i0[k] = (j0 << logC) + c;
 #pragma unroll
@@ -467,10 +493,10 @@ __device__ __forceinline__ void col_round(int logL, int logC, int logB, u32 b0, 
 // it (st); the rounds between go through the tile t in LDS.  IN_LDS: the
 // tile already holds the transform's words (filled by LDS DMA), and ld reads
 // them from there.
-template <int P, int R, bool DIF, bool IN_LDS, class Ld, class Fix, class St>
+template <int P, int R, bool DIF, bool IN_LDS, bool BIG, class Ld, class Fix, class St>
 __device__ __forceinline__ void col_round_io(bool first, bool last, u64* __restrict__ t, int logL, int logC, int logB,
-                                             u32 b0, const u64* __restrict__ W, int lq, int nthr, Ld ld, Fix fix,
-                                             St st) {
+                                             u32 b0, const u64* __restrict__ W, const u64* __restrict__ X, int F,
+                                             int lq, int nthr, Ld ld, Fix fix, St st) {
     // four data words of the first round in flight per thread: eight (two
     // radix-4 groups) took the kernel from 42 to 94 VGPRs and ran slower
     constexpr int GI = IN_LDS ? 1 : 4 >> R;
@@ -478,32 +504,35 @@ __device__ __forceinline__ void col_round_io(bool first, bool last, u64* __restr
     auto same = [](int, u64 v) { return v; };
     auto tst = [&](int e, u64 v) { t[e] = v; };
     if (first && last)
-        col_round<P, R, DIF, GI>(logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
+        col_round<P, R, DIF, GI, BIG>(logL, logC, logB, b0, W, X, F, lq, nthr, ld, fix, st);
     else if (first)
-        col_round<P, R, DIF, GI>(logL, logC, logB, b0, W, lq, nthr, ld, fix, tst);
+        col_round<P, R, DIF, GI, BIG>(logL, logC, logB, b0, W, X, F, lq, nthr, ld, fix, tst);
     else if (last)
-        col_round<P, R, DIF, 1>(logL, logC, logB, b0, W, lq, nthr, tld, same, st);
+        col_round<P, R, DIF, 1, BIG>(logL, logC, logB, b0, W, X, F, lq, nthr, tld, same, st);
     else
-        col_round<P, R, DIF, 1>(logL, logC, logB, b0, W, lq, nthr, tld, same, tst);
+        col_round<P, R, DIF, 1, BIG>(logL, logC, logB, b0, W, X, F, lq, nthr, tld, same, tst);
     if (!last) __syncthreads();
 }
 
 // The stages of a column tile (logL >= 1): DIF as radix-2^2 rounds over
 // hl = L/2 .. 1 with one radix-2 stage last when logL is odd; DIT as the
 // radix-2 stage first when logL is odd, then radix-2^2 rounds up to hl = L/2.
-template <int P, bool DIF, bool IN_LDS, class Ld, class Fix, class St>
+template <int P, bool DIF, bool IN_LDS, bool BIG, class Ld, class Fix, class St>
 __device__ __forceinline__ void tile_rounds(u64* __restrict__ t, int logL, int logC, int logB, u32 b0,
-                                            const u64* __restrict__ W, int nthr, Ld ld, Fix fix, St st) {
+                                            const u64* __restrict__ W, const u64* __restrict__ X, int F, int nthr,
+                                            Ld ld, Fix fix, St st) {
     const int nr = (logL >> 1) + (logL & 1);
     const bool odd = logL & 1;
     for (int i = 0; i < nr; i++) {
         const bool first = i == 0, last = i == nr - 1;
         const bool r1 = odd && (DIF ? last : first);
         if (r1) {
-            col_round_io<P, 1, DIF, IN_LDS>(first, last, t, logL, logC, logB, b0, W, 0, nthr, ld, fix, st);
+            col_round_io<P, 1, DIF, IN_LDS, BIG>(first, last, t, logL, logC, logB, b0, W, X, F, 0, nthr, ld, fix,
+                                                 st);
         } else {
             const int lq = DIF ? logL - 2 * (i + 1) : (logL & 1) + 2 * (i - (odd ? 1 : 0));
-            col_round_io<P, 2, DIF, IN_LDS>(first, last, t, logL, logC, logB, b0, W, lq, nthr, ld, fix, st);
+            col_round_io<P, 2, DIF, IN_LDS, BIG>(first, last, t, logL, logC, logB, b0, W, X, F, lq, nthr, ld, fix,
+                                                 st);
         }
     }
 }
@@ -522,6 +551,8 @@ struct ColArgs {
     const ProductDesc* items;
     ProductDesc one;
     const u64* tw[3];
+    const u64* twx[3];  // twiddles of half-spans >= 2^tw_full (launches with BIG set)
+    int tw_full;
     int log_n, s0, s1;
     int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
@@ -555,7 +586,7 @@ __device__ __forceinline__ void col_dma_fill(u64* t, int n, int nthr, const u64*
     __syncthreads();
 }
 
-template <int P, bool FWD, bool DMA>
+template <int P, bool FWD, bool DMA, bool BIG>
 __device__ void col_body(const ColArgs& A, u32 tile, int op) {
     extern __shared__ u64 lds[];
     const u32 prod = blockIdx.z;
@@ -582,7 +613,8 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
         auto tld = [&](int e) { return lds[e]; };
         if (!A.from_limbs) {
             col_dma_fill(lds, n, nthr, (const u64*)res, at, [](int) { return false; });
-            tile_rounds<P, FWD, true>(lds, logL, logC, logB, b0, A.tw[P], nthr, tld, same, put);
+            tile_rounds<P, FWD, true, BIG>(lds, logL, logC, logB, b0, A.tw[P], A.twx[P], A.tw_full, nthr, tld, same,
+                                           put);
         } else {
             // Pairs reaching past the operand's last word are not loaded: the
             // words past it are zero, and an odd length's last word (the
@@ -594,11 +626,13 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
                 if (g >= len) return 0ull;
                 return reduce_word<P>((g | 1) >= len ? limbs[g] : v);
             };
-            tile_rounds<P, FWD, true>(lds, logL, logC, logB, b0, A.tw[P], nthr, tld, val, put);
+            tile_rounds<P, FWD, true, BIG>(lds, logL, logC, logB, b0, A.tw[P], A.twx[P], A.tw_full, nthr, tld, val,
+                                           put);
         }
     } else if (!A.from_limbs) {
         auto get = [&](int e) { return res[at(e)]; };
-        tile_rounds<P, FWD, false>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, same, put);
+        tile_rounds<P, FWD, false, BIG>(lds, logL, logC, logB, b0, A.tw[P], A.twx[P], A.tw_full, nthr, get, same,
+                                        put);
     } else {
         // an empty operand reads the first twiddle and drops it
         const FMH_GLOBAL u64* __restrict__ src = len ? limbs : gmem(A.tw[P]);
@@ -608,7 +642,8 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
             return src[g < last ? g : last];
         };
         auto val = [&](int e, u64 v) { return at(e) < len ? reduce_word<P>(v) : 0ull; };
-        tile_rounds<P, FWD, false>(lds, logL, logC, logB, b0, A.tw[P], nthr, get, val, put);
+        tile_rounds<P, FWD, false, BIG>(lds, logL, logC, logB, b0, A.tw[P], A.twx[P], A.tw_full, nthr, get, val,
+                                        put);
     }
 }
 
@@ -621,7 +656,9 @@ __device__ void col_body(const ColArgs& A, u32 tile, int op) {
 // aligned; tiles of at least 2 adjacent lines).  Forward passes only: at
 // 2^25 points it took the forward passes from 1050 and 918 us to 1012 and
 // 859 us, and the inverse passes from 451 and 542 us to 445 and 559 us.
-template <bool FWD, bool DMA>
+// BIG: the pass has stages of half-span 2^tw_full or more (only the first
+// column pass of transforms above 2^tw_full points).
+template <bool FWD, bool DMA, bool BIG>
 __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
     const u32 per_tile = 3u * (u32)A.nops;
@@ -630,9 +667,9 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 prime = rem / (u32)A.nops;
     const int op = (int)(rem - prime * (u32)A.nops);
     switch (prime) {
-        case 0: col_body<0, FWD, DMA>(A, tile, op); break;
-        case 1: col_body<1, FWD, DMA>(A, tile, op); break;
-        default: col_body<2, FWD, DMA>(A, tile, op); break;
+        case 0: col_body<0, FWD, DMA, BIG>(A, tile, op); break;
+        case 1: col_body<1, FWD, DMA, BIG>(A, tile, op); break;
+        default: col_body<2, FWD, DMA, BIG>(A, tile, op); break;
     }
 }
 
@@ -1551,50 +1588,101 @@ __global__ void k_tw_low(TwArgs A) {
     tab[i] = tab[half + t * (half >> lh)];
 }
 
-// Builds tables of 2^log_t words and then swaps them in.  The tables they
-// replace are never freed: work already enqueued, and graphs captured with
-// them, keep valid pointers (tables only grow, so at most half of the
-// memory held is retired).
+int tw_full_log() {
+    static const int F = [] {
+        const char* v = getenv("FMH_TW_FULL_LOG");
+        const int f = v && *v ? atoi(v) : kTwFullLog;
+        return std::max(kMaxRowTileLog, std::min(f, kMaxLogN));
+    }();
+    return F;
+}
+
+// Builds the tables for transforms of up to 2^log_t points and then swaps
+// them in.  The full tables (2^min(log_t, F) words, built on the device) are
+// rebuilt only when they grow; the extension tables of the half-spans >= 2^F
+// (2^(log_t - F + 1) words, at most 2^(31 - F): 64 words per table with
+// F = 25) are computed on the host.  Tables that are replaced are never
+// freed: work already enqueued, and graphs captured with them, keep valid
+// pointers (tables only grow, so at most half of the memory held is
+// retired).
 hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
     if (log_t < 1) log_t = 1;
-    const u64 T = 1ull << log_t;
-    u64* fwd[3] = {nullptr, nullptr, nullptr};
-    u64* inv[3] = {nullptr, nullptr, nullptr};
-    TwArgs A{};
-    A.log_t = log_t;
-    for (int i = 0; i < 3; i++) {
-        hipError_t e = hipMalloc(&fwd[i], T * sizeof(u64));
-        if (e == hipSuccess) e = hipMalloc(&inv[i], T * sizeof(u64));
-        if (e != hipSuccess) {
-            for (int j = 0; j <= i; j++) {
-                if (fwd[j]) (void)hipFree(fwd[j]);
-                if (inv[j]) (void)hipFree(inv[j]);
-            }
-            return e;
+    const int F = tw_full_log();
+    const int lf = std::min(log_t, F);
+    u64* fwd[3] = {tw.fwd[0], tw.fwd[1], tw.fwd[2]};
+    u64* inv[3] = {tw.inv[0], tw.inv[1], tw.inv[2]};
+    u64* fwd_x[3] = {nullptr, nullptr, nullptr};
+    u64* inv_x[3] = {nullptr, nullptr, nullptr};
+    std::vector<void*> made;
+    auto fail = [&](hipError_t e) {
+        for (void* q : made) (void)hipFree(q);
+        return e;
+    };
+    if (tw.log_full != lf) {
+        const u64 T = 1ull << lf;
+        TwArgs A{};
+        A.log_t = lf;
+        for (int i = 0; i < 3; i++) {
+            hipError_t e = hipMalloc(&fwd[i], T * sizeof(u64));
+            if (e == hipSuccess) made.push_back(fwd[i]);
+            if (e == hipSuccess) e = hipMalloc(&inv[i], T * sizeof(u64));
+            if (e == hipSuccess) made.push_back(inv[i]);
+            if (e != hipSuccess) return fail(e);
+            const u64 p = kP[i];
+            const u64 root = h_powmod(kG[i], (p - 1) >> lf, p);
+            const u64 iroot = h_powmod(root, p - 2, p);
+            A.tab[i] = fwd[i];
+            A.tab[3 + i] = inv[i];
+            A.root[i] = h_mont(root, p);
+            A.root[3 + i] = h_mont(iroot, p);
+            A.one[i] = h_mont(1, p);
         }
-        const u64 p = kP[i];
-        const u64 root = h_powmod(kG[i], (p - 1) >> log_t, p);
-        const u64 iroot = h_powmod(root, p - 2, p);
-        A.tab[i] = fwd[i];
-        A.tab[3 + i] = inv[i];
-        A.root[i] = h_mont(root, p);
-        A.root[3 + i] = h_mont(iroot, p);
-        A.one[i] = h_mont(1, p);
+        const u64 half = T / 2;
+        dim3 grid((unsigned)((half + 255) / 256), 6);
+        hipLaunchKernelGGL(k_tw_top, grid, dim3(256), 0, s, A);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(e);
+        hipLaunchKernelGGL(k_tw_low, grid, dim3(256), 0, s, A);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(e);
     }
-    const u64 half = T / 2;
-    dim3 grid((unsigned)((half + 255) / 256), 6);
-    hipLaunchKernelGGL(k_tw_top, grid, dim3(256), 0, s, A);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tw_low, grid, dim3(256), 0, s, A);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
+    if (log_t > F) {
+        // entry [K + t] = w_{2h}^t (t < K) for h = K 2^(F-1), K = 2 .. 2^(log_t - F)
+        const u64 X = 2ull << (log_t - F);
+        std::vector<u64> hx(X), hix(X);
+        for (int i = 0; i < 3; i++) {
+            const u64 p = kP[i];
+            hx[0] = hx[1] = hix[0] = hix[1] = 0;
+            for (u64 K = 2; K < X; K <<= 1) {
+                const int order_log = F + ceil_log2(K);  // 2h = 2^(F + log K)
+                const u64 root = h_powmod(kG[i], (p - 1) >> order_log, p);
+                const u64 iroot = h_powmod(root, p - 2, p);
+                u64 r = 1, ir = 1;
+                for (u64 t = 0; t < K; t++) {
+                    hx[K + t] = h_mont(r, p);
+                    hix[K + t] = h_mont(ir, p);
+                    r = (u64)(((u128)r * root) % p);
+                    ir = (u64)(((u128)ir * iroot) % p);
+                }
+            }
+            hipError_t e = hipMalloc(&fwd_x[i], X * sizeof(u64));
+            if (e == hipSuccess) made.push_back(fwd_x[i]);
+            if (e == hipSuccess) e = hipMalloc(&inv_x[i], X * sizeof(u64));
+            if (e == hipSuccess) made.push_back(inv_x[i]);
+            if (e == hipSuccess) e = hipMemcpy(fwd_x[i], hx.data(), X * sizeof(u64), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(inv_x[i], hix.data(), X * sizeof(u64), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(e);
+        }
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(e);
     for (int i = 0; i < 3; i++) {
         tw.fwd[i] = fwd[i];
         tw.inv[i] = inv[i];
+        tw.fwd_x[i] = fwd_x[i];
+        tw.inv_x[i] = inv_x[i];
     }
+    tw.log_full = lf;
     tw.log_t = log_t;
     return hipSuccess;
 }
@@ -1617,8 +1705,9 @@ static const CrtConsts& crt_table(int log_n) {
 // 160 KiB per workgroup, static LDS included; kernels default to 64 KiB
 // unless the limit is raised.  Launches that would exceed it are refused
 // with an error instead of being sent.
-// k_col<true, false>, k_col<false, false>, k_row<4>, k_row<3>, k_col<true, true>
-static size_t g_max_dyn[5] = {0, 0, 0, 0, 0};
+// k_col<true, false, false>, k_col<false, false, false>, k_row<4>, k_row<3>,
+// k_col<true, true, false>, then the BIG instances of the three k_col
+static size_t g_max_dyn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
 static size_t raise_lds_limit(const void* f) {
     constexpr size_t kDefault = 64 * 1024;
@@ -1636,11 +1725,14 @@ void prepare_kernels() {
     (void)crt_table(0);
     static std::once_flag once;
     std::call_once(once, [] {
-        g_max_dyn[0] = raise_lds_limit((const void*)k_col<true, false>);
-        g_max_dyn[1] = raise_lds_limit((const void*)k_col<false, false>);
+        g_max_dyn[0] = raise_lds_limit((const void*)k_col<true, false, false>);
+        g_max_dyn[1] = raise_lds_limit((const void*)k_col<false, false, false>);
         g_max_dyn[2] = raise_lds_limit((const void*)k_row<4>);
         g_max_dyn[3] = raise_lds_limit((const void*)k_row<3>);
-        g_max_dyn[4] = raise_lds_limit((const void*)k_col<true, true>);
+        g_max_dyn[4] = raise_lds_limit((const void*)k_col<true, true, false>);
+        g_max_dyn[5] = raise_lds_limit((const void*)k_col<true, false, true>);
+        g_max_dyn[6] = raise_lds_limit((const void*)k_col<false, false, true>);
+        g_max_dyn[7] = raise_lds_limit((const void*)k_col<true, true, true>);
     });
 }
 
@@ -1883,6 +1975,40 @@ static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const Produc
     return true;
 }
 
+// One column pass: the k_col instance for its direction, its fill (LDS DMA or
+// not) and whether its stages reach half-spans of 2^tw.log_full or more.
+static hipError_t launch_col(ColArgs& C, const Twiddles& tw, bool fwd, bool dma, dim3 grid, size_t lds,
+                             hipStream_t s) {
+    for (int i = 0; i < 3; i++) {
+        C.tw[i] = fwd ? tw.fwd[i] : tw.inv[i];
+        C.twx[i] = fwd ? tw.fwd_x[i] : tw.inv_x[i];
+    }
+    C.tw_full = tw.log_full;
+    const bool big = C.log_n - 1 - C.s0 >= tw.log_full;  // largest half-span 2^(log_n - 1 - s0)
+    if (big && !C.twx[0]) return hipErrorInvalidValue;
+    const int k = fwd ? (dma ? 4 : 0) : 1;
+    if (lds > g_max_dyn[big ? (k == 4 ? 7 : 5 + k) : k]) return hipErrorInvalidValue;
+    const char* name = fwd ? "col_fwd" : "col_inv";
+    const dim3 block(C.threads);
+    if (fwd && dma) {
+        if (big)
+            FMH_LAUNCH(name, (k_col<true, true, true>), grid, block, lds, s, C);
+        else
+            FMH_LAUNCH(name, (k_col<true, true, false>), grid, block, lds, s, C);
+    } else if (fwd) {
+        if (big)
+            FMH_LAUNCH(name, (k_col<true, false, true>), grid, block, lds, s, C);
+        else
+            FMH_LAUNCH(name, (k_col<true, false, false>), grid, block, lds, s, C);
+    } else {
+        if (big)
+            FMH_LAUNCH(name, (k_col<false, false, true>), grid, block, lds, s, C);
+        else
+            FMH_LAUNCH(name, (k_col<false, false, false>), grid, block, lds, s, C);
+    }
+    return hipSuccess;
+}
+
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
@@ -1898,7 +2024,6 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         ColArgs C{};
         C.items = items;
         C.one = d0;
-        for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
@@ -1910,12 +2035,8 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
         const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &j.items[first], count, nops);
-        if (lds > g_max_dyn[dma ? 4 : 0]) return hipErrorInvalidValue;
-        const dim3 grid((unsigned)(tiles * 3 * nops), 1, count);
-        if (dma)
-            FMH_LAUNCH("col_fwd", (k_col<true, true>), grid, dim3(tu.col_threads), lds, s, C);
-        else
-            FMH_LAUNCH("col_fwd", (k_col<true, false>), grid, dim3(tu.col_threads), lds, s, C);
+        const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3 * nops), 1, count), lds, s);
+        if (e != hipSuccess) return e;
     }
     {
         RowArgs R{};
@@ -1951,7 +2072,6 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         ColArgs C{};
         C.items = items;
         C.one = d0;
-        for (int i = 0; i < 3; i++) C.tw[i] = tw.inv[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
@@ -1962,9 +2082,8 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
-        if (lds > g_max_dyn[1]) return hipErrorInvalidValue;
-        FMH_LAUNCH("col_inv", (k_col<false, false>), dim3((unsigned)(tiles * 3), 1, count), dim3(tu.col_threads), lds,
-                   s, C);
+        const hipError_t e = launch_col(C, tw, false, false, dim3((unsigned)(tiles * 3), 1, count), lds, s);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -2047,7 +2166,6 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     for (int pi = 0; pi < pl.n_col; pi++) {
         ColArgs C{};
         C.one = d;
-        for (int i = 0; i < 3; i++) C.tw[i] = tw.fwd[i];
         C.log_n = log_n;
         C.s0 = pl.col_s[pi];
         C.s1 = pl.col_s[pi + 1];
@@ -2059,13 +2177,8 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
         const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &d, 1, 1);
-        if (lds > g_max_dyn[dma ? 4 : 0]) return hipErrorInvalidValue;
-        if (dma)
-            FMH_LAUNCH("col_fwd", (k_col<true, true>), dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds,
-                       s, C);
-        else
-            FMH_LAUNCH("col_fwd", (k_col<true, false>), dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.col_threads), lds,
-                       s, C);
+        const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3), 1, 1), lds, s);
+        if (e != hipSuccess) return e;
     }
     RowArgs R{};
     R.one = d;

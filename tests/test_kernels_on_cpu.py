@@ -30,7 +30,10 @@ def test_kernels_emulated_on_cpu(tmp_path):
     env = dict(os.environ, LD_PRELOAD=_asan_runtime(), ASAN_OPTIONS="detect_leaks=0",
                # small tiles: transforms above 256 points take two or more
                # column passes, so short operands reach every pass schedule
-               FMH_TUNE="col_threads=64,row_threads=64,row_tile=8,row_maxl=7,col_maxl=5")
+               FMH_TUNE="col_threads=64,row_threads=64,row_tile=8,row_maxl=7,col_maxl=5",
+               # full twiddle tables up to 2^13 points: the first column pass of
+               # longer transforms forms its twiddles from two entries
+               FMH_TW_FULL_LOG="13")
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu", "emu_check.py"),
                         str(tmp_path / "libfmh_emu.so")], capture_output=True, text=True, env=env,
                        timeout=1200)
