@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <map>
 #include <mutex>
@@ -33,18 +34,25 @@ struct Buffer {
 
 // Memory copies spread over a few persistent host threads (the calling
 // thread takes one part), for moving large pageable buffers into and out of
-// pinned staging memory.
+// pinned staging memory.  A staged transfer makes one such copy per chunk,
+// tens of microseconds apart; a worker that went to sleep would take about as
+// long to wake, so workers spin on the generation counter for a while after
+// each copy before they block.
+#ifndef FMH_COPY_THREADS
+#define FMH_COPY_THREADS 7
+#endif
 class CopyPool {
    public:
     CopyPool() {
         const unsigned hw = std::thread::hardware_concurrency();
-        const int workers = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+        const int workers = (int)std::min((unsigned)FMH_COPY_THREADS, hw > 1 ? hw - 1 : 0u);
         for (int i = 0; i < workers; i++) th_.emplace_back([this, i] { work(i + 1); });
     }
     ~CopyPool() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
+            gen_.fetch_add(1);
         }
         cv_.notify_all();
         for (auto& t : th_) t.join();
@@ -55,21 +63,26 @@ class CopyPool {
             return;
         }
         std::lock_guard<std::mutex> one(call_mu_);  // one split copy at a time
-        std::unique_lock<std::mutex> lk(mu_);
         dst_ = (char*)dst;
         src_ = (const char*)src;
         n_ = n;
         parts_ = (int)th_.size() + 1;
-        pending_ = (int)th_.size();
-        gen_++;
-        lk.unlock();
+        pending_.store((int)th_.size());
+        {
+            std::lock_guard<std::mutex> lk(mu_);  // orders with a worker about to sleep
+            gen_.fetch_add(1);
+        }
         cv_.notify_all();
         part(0);
-        lk.lock();
-        done_.wait(lk, [&] { return pending_ == 0; });
+        while (pending_.load() != 0) cpu_relax();
     }
 
    private:
+    static void cpu_relax() {
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#endif
+    }
     void part(int id) {
         const size_t per = (n_ / parts_ + 63) & ~(size_t)63;
         const size_t b = std::min(n_, per * (size_t)id), e = std::min(n_, b + per);
@@ -77,25 +90,28 @@ class CopyPool {
     }
     void work(int id) {
         unsigned long long seen = 0;
-        std::unique_lock<std::mutex> lk(mu_);
         while (true) {
-            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            // spin about a millisecond for the next copy, then sleep
+            for (int i = 0; i < 20000 && gen_.load() == seen; i++) cpu_relax();
+            if (gen_.load() == seen) {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_.load() != seen; });
+            }
+            seen = gen_.load();
             if (stop_) return;
-            seen = gen_;
-            lk.unlock();
             part(id);
-            lk.lock();
-            if (--pending_ == 0) done_.notify_one();
+            pending_.fetch_sub(1);
         }
     }
     std::vector<std::thread> th_;
     std::mutex call_mu_, mu_;
-    std::condition_variable cv_, done_;
+    std::condition_variable cv_;
     char* dst_ = nullptr;
     const char* src_ = nullptr;
     size_t n_ = 0;
-    int parts_ = 1, pending_ = 0;
-    unsigned long long gen_ = 0;
+    int parts_ = 1;
+    std::atomic<int> pending_{0};
+    std::atomic<unsigned long long> gen_{0};
     bool stop_ = false;
 };
 
@@ -107,13 +123,17 @@ CopyPool& copy_pool() {
 // Host-pointer calls move data through a ring of kRing pinned buffers of
 // kPinChunk bytes; event i marks the last transfer that used buffer i.
 #ifndef FMH_PIN_CHUNK
-#define FMH_PIN_CHUNK (4u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
+#define FMH_PIN_CHUNK (8u << 20)  // the CPU emulation builds with a small chunk to reach the staged path
 #endif
 constexpr size_t kPinChunk = FMH_PIN_CHUNK;
+#ifndef FMH_OVERLAP_BYTES
+#define FMH_OVERLAP_BYTES (4u << 20)  // the CPU emulation builds with a small value to reach both paths
+#endif
+constexpr size_t kOverlapBytes = FMH_OVERLAP_BYTES;
 #ifndef FMH_PIN_RING
 #define FMH_PIN_RING 3
 #endif
-constexpr int kRing = FMH_PIN_RING;  // 4 MiB DMA chunks run at 51 GB/s, 2 MiB ones at 45 GB/s
+constexpr int kRing = FMH_PIN_RING;  // DMA from pinned memory: 8 MiB chunks run at 54 GB/s, 4 MiB at 51, 2 MiB at 45
 
 // Device scratch of fmh_mul_device_noalloc: the workspace (residues of both
 // operands for the three primes, per-limb carry maps, per-chunk maps and
@@ -134,8 +154,13 @@ static_assert((1ull << kNoAllocLogN) == FMH_NOALLOC_MAX_NTT_LEN, "fmh.h states t
 // the operands and the output, and the pinned ring.  Calls take a free slot
 // of their device (or make one), so concurrent callers run their transfers
 // and kernels side by side instead of one after the other.
+// Transfers run on their own stream, so the second operand's upload overlaps
+// the first operand's forward column pass on the compute stream.
 struct HostSlot {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // kernels
+    hipStream_t xfer = nullptr;    // uploads and downloads
+    hipEvent_t up[2] = {};         // operand i is on the device
+    hipEvent_t done = nullptr;     // the product is in stage_out
     Buffer stage_a, stage_b, stage_out;
     void* pin[kRing] = {};
     hipEvent_t pin_ev[kRing] = {};
@@ -229,7 +254,7 @@ struct Staging {
 std::map<hipStream_t, Staging> g_stage;
 
 // Enqueue a set of products on stream s; callers hold g_mu.
-int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
+int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s, hipEvent_t wait_b = nullptr) {
     Job job;
     if (!products_valid(in, count)) return set_err("NULL device pointer for a non-empty operand or output");
     const int rc = job_build(job, in, count);
@@ -264,15 +289,15 @@ int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
         if (e == hipSuccess) e = hipEventRecord(st.done, s);
         if (e != hipSuccess) return set_err("descriptor upload", e);
     }
-    e = job_launch(job, c.tw, s);
+    e = job_launch(job, c.tw, s, wait_b);
     if (e != hipSuccess) return set_err("kernel launch", e);
     return 0;
 }
 
 int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t wait_b = nullptr) {
     const ProductIn in{a, na, square ? nullptr : b, square ? na : nb, out};
-    return run_job(c, &in, 1, s);
+    return run_job(c, &in, 1, s, wait_b);
 }
 
 int ensure_pinned(HostSlot& h) {
@@ -331,20 +356,24 @@ std::vector<std::pair<size_t, size_t>> d2h_chunks(size_t bytes) {
 // costs hundreds of microseconds even for small buffers), every segment in
 // one pipeline: the host threads fill buffer k % kRing while the chunks
 // before it move over DMA, so host copies and DMA overlap across segments.
-int h2d_staged(HostSlot& h, const Segment* seg, int nseg) {
+// Runs on stream xs; segment g's last DMA is followed by a record of h.up[g].
+int h2d_staged(HostSlot& h, hipStream_t xs, const Segment* seg, int nseg) {
     if (ensure_pinned(h)) return -1;
     size_t k = 0;
-    for (int g = 0; g < nseg; g++)
+    for (int g = 0; g < nseg; g++) {
         for (const auto& ch : h2d_chunks(seg[g].bytes, k == 0)) {
             const size_t off = ch.first, n = ch.second;
             const int i = (int)(k++ % kRing);
             hipError_t e = hipEventSynchronize(h.pin_ev[i]);  // the DMA that last read buffer i
             if (e != hipSuccess) return set_err("hipEventSynchronize", e);
             copy_pool().copy(h.pin[i], (const char*)seg[g].host + off, n);
-            e = hipMemcpyAsync((char*)seg[g].dev + off, h.pin[i], n, hipMemcpyHostToDevice, h.stream);
-            if (e == hipSuccess) e = hipEventRecord(h.pin_ev[i], h.stream);
+            e = hipMemcpyAsync((char*)seg[g].dev + off, h.pin[i], n, hipMemcpyHostToDevice, xs);
+            if (e == hipSuccess) e = hipEventRecord(h.pin_ev[i], xs);
             if (e != hipSuccess) return set_err("hipMemcpyAsync H2D", e);
         }
+        const hipError_t e = hipEventRecord(h.up[g], xs);
+        if (e != hipSuccess) return set_err("hipEventRecord", e);
+    }
     return 0;
 }
 
@@ -352,14 +381,14 @@ int h2d_staged(HostSlot& h, const Segment* seg, int nseg) {
 // the host threads copy the oldest one out of its buffer; each buffer's next
 // DMA is enqueued as soon as it has been emptied.  Returns after the last
 // byte.
-int d2h_staged(HostSlot& h, void* dst, const void* src, size_t bytes) {
+int d2h_staged(HostSlot& h, hipStream_t xs, void* dst, const void* src, size_t bytes) {
     if (ensure_pinned(h)) return -1;
     const std::vector<std::pair<size_t, size_t>> ch = d2h_chunks(bytes);
     const size_t chunks = ch.size();
     auto issue = [&](size_t k) {
         hipError_t r = hipMemcpyAsync(h.pin[k % kRing], (const char*)src + ch[k].first, ch[k].second,
-                                      hipMemcpyDeviceToHost, h.stream);
-        if (r == hipSuccess) r = hipEventRecord(h.pin_ev[k % kRing], h.stream);
+                                      hipMemcpyDeviceToHost, xs);
+        if (r == hipSuccess) r = hipEventRecord(h.pin_ev[k % kRing], xs);
         return r;
     };
     hipError_t e = hipSuccess;
@@ -399,8 +428,11 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     if (!h) {
         h = new HostSlot();
         hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->xfer, hipStreamNonBlocking);
+        for (int i = 0; e == hipSuccess && i < 2; i++) e = hipEventCreateWithFlags(&h->up[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->done, hipEventDisableTiming);
         if (e != hipSuccess) {
-            set_err("hipStreamCreate", e);
+            set_err("host-call slot", e);
             die(fn);
         }
     }
@@ -413,12 +445,38 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     u64* dout = (u64*)h->stage_out.ptr;
     // both operands are read before out is written, so out may alias them
     const Segment seg[2] = {{da, (void*)in1, len1 * sizeof(u64)}, {db, (void*)in2, square ? 0 : len2 * sizeof(u64)}};
-    if (h2d_staged(*h, seg, 2)) die(fn);
+    // A second operand of at least kOverlapBytes moves on the transfer stream
+    // while the first operand's forward column pass runs on the compute
+    // stream (2^20 x 2^20 limbs: 980 -> 958 us); below that the cross-stream
+    // waits cost more than they hide (2^16 x 2^16: 159 -> 170 us), and the
+    // whole call runs on the compute stream.
+    const bool overlap = !square && len2 * sizeof(u64) >= kOverlapBytes;
+    const hipStream_t xs = overlap ? h->xfer : h->stream;
+    if (h2d_staged(*h, xs, seg, 2)) die(fn);
     {
         std::lock_guard<std::mutex> lk(g_mu);  // the device state: twiddles, per-stream workspaces
-        if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, h->stream)) die(fn);
+        hipError_t e = hipSuccess;
+        if (overlap) {
+            // the first operand's forward pass may start before the second
+            // operand has arrived (job_launch waits for up[1] where it must)
+            e = hipStreamWaitEvent(h->stream, h->up[0], 0);
+            if (e != hipSuccess) {
+                set_err("hipStreamWaitEvent", e);
+                die(fn);
+            }
+        }
+        if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, h->stream, overlap ? h->up[1] : nullptr))
+            die(fn);
+        if (overlap) {
+            e = hipEventRecord(h->done, h->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->xfer, h->done, 0);
+            if (e != hipSuccess) {
+                set_err("hipEventRecord", e);
+                die(fn);
+            }
+        }
     }
-    if (d2h_staged(*h, out, dout, out_len * sizeof(u64))) die(fn);
+    if (d2h_staged(*h, xs, out, dout, out_len * sizeof(u64))) die(fn);
     std::lock_guard<std::mutex> lk(g_mu);
     c->free_slots.push_back(h);
 }

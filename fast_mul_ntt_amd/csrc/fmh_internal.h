@@ -225,7 +225,10 @@ void job_pack(const Job& j, void* host);
 // Copies the descriptors with a synchronous-source hipMemcpyAsync.
 hipError_t job_upload(const Job& j, hipStream_t s);
 // Enqueues the kernels (no copies, no allocation: capturable in a graph).
-hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s);
+// wait_b (host-pointer calls): an event that marks the second operand's
+// upload; a single multiply transforms its first operand before waiting for
+// it, any other job waits for it first.
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, hipEvent_t wait_b = nullptr);
 
 hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s);
 // Plain device copy of n words (HBM calibration kernel for profilers).

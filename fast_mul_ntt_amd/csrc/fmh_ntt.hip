@@ -557,6 +557,7 @@ struct ColArgs {
     int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
     int nops;     // operands transformed per product: 2 (multiply) or 1
+    int op0;      // operand of slot operand 0 (a pass over the second operand alone: 1)
     int threads;  // workgroup size
 };
 
@@ -665,7 +666,7 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 tile = slot / per_tile;
     const u32 rem = slot - tile * per_tile;
     const u32 prime = rem / (u32)A.nops;
-    const int op = (int)(rem - prime * (u32)A.nops);
+    const int op = A.op0 + (int)(rem - prime * (u32)A.nops);
     switch (prime) {
         case 0: col_body<0, FWD, DMA, BIG>(A, tile, op); break;
         case 1: col_body<1, FWD, DMA, BIG>(A, tile, op); break;
@@ -1963,11 +1964,12 @@ hipError_t job_upload(const Job& j, hipStream_t s) {
 // Whether a forward column pass over the products d[0, count) may fill its
 // tiles by LDS DMA: tiles of at least two adjacent lines, and for the first
 // pass every operand it reads (nops of each product) 16-byte aligned.
-static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const ProductDesc* d, size_t count, int nops) {
+static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const ProductDesc* d, size_t count, int nops,
+                       int op0 = 0) {
     if (!tu.col_dma || logC < 1) return false;
     if (!from_limbs) return true;
     for (size_t i = 0; i < count; i++)
-        for (int op = 0; op < nops; op++) {
+        for (int op = op0; op < op0 + nops; op++) {
             const u64 len = op ? d[i].nb : d[i].na;
             const uintptr_t p = (uintptr_t)(op ? d[i].b : d[i].a);
             if (len && (p & 15)) return false;
@@ -2011,7 +2013,10 @@ static hipError_t launch_col(ColArgs& C, const Twiddles& tw, bool fwd, bool dma,
 
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
-static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
+// wait_b (not null): the first forward pass of a multiply runs one launch per
+// operand, the second operand's after wait_b.
+static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s,
+                               hipEvent_t wait_b) {
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
     const Tuning tu = tuning(log_n, count >= kBatchedClass);
@@ -2034,6 +2039,20 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
+        if (pi == 0 && wait_b && nops == 2) {
+            for (int op = 0; op < 2; op++) {
+                if (op == 1) {
+                    const hipError_t we = hipStreamWaitEvent(s, wait_b, 0);
+                    if (we != hipSuccess) return we;
+                }
+                C.nops = 1;
+                C.op0 = op;
+                const bool dma = col_dma_ok(tu, C.logC, true, &j.items[first], count, 1, op);
+                const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3), 1, count), lds, s);
+                if (e != hipSuccess) return e;
+            }
+            continue;
+        }
         const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &j.items[first], count, nops);
         const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3 * nops), 1, count), lds, s);
         if (e != hipSuccess) return e;
@@ -2088,9 +2107,20 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     return hipSuccess;
 }
 
-hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, hipEvent_t wait_b) {
     prepare_kernels();
     if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
+    if (wait_b) {
+        // only one multiply whose transform has a column pass splits its
+        // first pass; everything else waits for both operands first
+        const bool split = j.items.size() == 1 && j.items[0].kind == kKindMul && make_plan((int)j.items[0].log_n,
+            tuning((int)j.items[0].log_n)).n_col > 0;
+        if (!split) {
+            const hipError_t we = hipStreamWaitEvent(s, wait_b, 0);
+            if (we != hipSuccess) return we;
+            wait_b = nullptr;
+        }
+    }
     for (const auto& z : j.zeros) {
         hipError_t e = enqueue_zero(z.first, z.second, s);
         if (e != hipSuccess) return e;
@@ -2104,7 +2134,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
         while (last < j.items.size() && last - first < kMaxClass && j.items[last].log_n == j.items[first].log_n &&
                j.items[last].kind == j.items[first].kind)
             last++;
-        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s);
+        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s, wait_b);
         if (e != hipSuccess) return e;
         first = last;
     }

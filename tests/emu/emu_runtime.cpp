@@ -188,6 +188,7 @@ hipError_t hipStreamDestroy(hipStream_t s) {
     return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }  // streams run eagerly
 hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new emu_event();
     return hipSuccess;
