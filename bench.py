@@ -262,7 +262,7 @@ def run_serve(args, fm) -> dict:
             fm.abs_mul64(a, src_host[o2[i]:o2[i] + l2[i]])
     host_rate = len(host_idx) / (time.perf_counter() - t0)
 
-    srv = fm.Server(pool_bytes=args.pool_gib << 30, streams=4)
+    srv = fm.Server(pool_bytes=args.pool_gib << 30, streams=args.serve_streams)
     check_every = max(1, count // 40)
     checked = {}
 
@@ -347,6 +347,7 @@ def run_serve(args, fm) -> dict:
         "capacity_classes_by_device_time": top_classes(cap_classes),
         "classes_by_device_time": top_classes(classes),
         "pool_gib": args.pool_gib,
+        "serve_streams": args.serve_streams,
         "pool_peak_gib": round(st["pool_peak_bytes"] / 2**30, 3),
         "submit_lateness_p99_ms": round(float(np.percentile(late, 99)) * 1e3, 3),
         "checked_vs_plain": len(checked),
@@ -483,11 +484,20 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
     # buffers and host threads are made on first use).
     for _ in range(2):
         fm.abs_mul64(a, b)
+    # then timed: fresh output arrays (their pages are first touched inside
+    # the call), and one output array reused
     t_host = []
     for _ in range(7):
         h0 = time.perf_counter()
         host_out = fm.abs_mul64(a, b)
         t_host.append(time.perf_counter() - h0)
+    if not np.array_equal(host_out, got):
+        raise AssertionError("host-pointer abs_mul64 differs from the device-pointer result")
+    t_host_reuse = []
+    for _ in range(7):
+        h0 = time.perf_counter()
+        fm.abs_mul64(a, b, host_out)
+        t_host_reuse.append(time.perf_counter() - h0)
     if not np.array_equal(host_out, got):
         raise AssertionError("host-pointer abs_mul64 differs from the device-pointer result")
 
@@ -558,6 +568,8 @@ def run_single(args, fm, n1: int, n2: int, seed_a: int, seed_b: int, metric: str
         "kernel_sum_us": round(kernel_sum_us, 2),
         "host_abi_us": round(min(t_host) * 1e6, 1),
         "host_abi_median_us": round(float(np.median(t_host)) * 1e6, 1),
+        "host_abi_reused_out_us": round(min(t_host_reuse) * 1e6, 1),
+        "host_abi_reused_out_median_us": round(float(np.median(t_host_reuse)) * 1e6, 1),
     }
 
     if not args.no_cpu_baseline:
@@ -610,6 +622,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=20000, help="serve: number of requests")
     ap.add_argument("--load", type=float, default=0.7, help="serve: offered rate / capacity")
     ap.add_argument("--pool-gib", type=int, default=64, help="serve: device pool (GiB)")
+    ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams on which groups overlap")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus})",
