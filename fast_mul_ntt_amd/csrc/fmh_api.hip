@@ -62,6 +62,24 @@ class CopyPool {
             memcpy(dst, src, n);
             return;
         }
+        run(dst, src, n);
+    }
+    // Writes a zero byte into every 4 KiB page of [dst, dst + n), so that a
+    // fresh buffer's page faults are taken here rather than during a later
+    // copy into it.
+    void touch(void* dst, size_t n) {
+        if (th_.empty() || n < (1u << 20)) {
+            touch_range((char*)dst, n);
+            return;
+        }
+        run(dst, nullptr, n);
+    }
+
+   private:
+    static void touch_range(char* p, size_t n) {
+        for (size_t o = 0; o < n; o += 4096) ((volatile char*)p)[o] = 0;
+    }
+    void run(void* dst, const void* src, size_t n) {
         std::lock_guard<std::mutex> one(call_mu_);  // one split copy at a time
         dst_ = (char*)dst;
         src_ = (const char*)src;
@@ -77,7 +95,6 @@ class CopyPool {
         while (pending_.load() != 0) cpu_relax();
     }
 
-   private:
     static void cpu_relax() {
 #if defined(__x86_64__)
         __builtin_ia32_pause();
@@ -86,7 +103,11 @@ class CopyPool {
     void part(int id) {
         const size_t per = (n_ / parts_ + 63) & ~(size_t)63;
         const size_t b = std::min(n_, per * (size_t)id), e = std::min(n_, b + per);
-        if (e > b) memcpy(dst_ + b, src_ + b, e - b);
+        if (e <= b) return;
+        if (src_)
+            memcpy(dst_ + b, src_ + b, e - b);
+        else
+            touch_range(dst_ + b, e - b);
     }
     void work(int id) {
         unsigned long long seen = 0;
@@ -253,9 +274,9 @@ struct Staging {
 };
 std::map<hipStream_t, Staging> g_stage;
 
-// Enqueue a set of products on stream s; callers hold g_mu.
-int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s, hipEvent_t wait_b = nullptr) {
-    Job job;
+// Builds a job of products for stream s: twiddles, the stream's workspace,
+// descriptors uploaded; callers hold g_mu.
+int prepare_job(DevCtx& c, Job& job, const ProductIn* in, size_t count, hipStream_t s) {
     if (!products_valid(in, count)) return set_err("NULL device pointer for a non-empty operand or output");
     const int rc = job_build(job, in, count);
     if (rc == -2) return set_err("product longer than the spectrum's transform length");
@@ -289,15 +310,22 @@ int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s, hipEven
         if (e == hipSuccess) e = hipEventRecord(st.done, s);
         if (e != hipSuccess) return set_err("descriptor upload", e);
     }
-    e = job_launch(job, c.tw, s, wait_b);
+    return 0;
+}
+
+// Enqueue a set of products on stream s; callers hold g_mu.
+int run_job(DevCtx& c, const ProductIn* in, size_t count, hipStream_t s) {
+    Job job;
+    if (prepare_job(c, job, in, count, s)) return -1;
+    const hipError_t e = job_launch(job, c.tw, s);
     if (e != hipSuccess) return set_err("kernel launch", e);
     return 0;
 }
 
 int run_product(DevCtx& c, const u64* a, u64 na, const u64* b, u64 nb, bool square, u64* out,
-                hipStream_t s, hipEvent_t wait_b = nullptr) {
+                hipStream_t s) {
     const ProductIn in{a, na, square ? nullptr : b, square ? na : nb, out};
-    return run_job(c, &in, 1, s, wait_b);
+    return run_job(c, &in, 1, s);
 }
 
 int ensure_pinned(HostSlot& h) {
@@ -356,10 +384,11 @@ std::vector<std::pair<size_t, size_t>> d2h_chunks(size_t bytes) {
 // costs hundreds of microseconds even for small buffers), every segment in
 // one pipeline: the host threads fill buffer k % kRing while the chunks
 // before it move over DMA, so host copies and DMA overlap across segments.
-// Runs on stream xs; segment g's last DMA is followed by a record of h.up[g].
-int h2d_staged(HostSlot& h, hipStream_t xs, const Segment* seg, int nseg) {
+// Runs on stream xs; k counts the chunks of the transfer (it continues
+// across calls that upload parts of one transfer); mark (nullable) is
+// recorded after the last DMA.
+int h2d_staged(HostSlot& h, hipStream_t xs, const Segment* seg, int nseg, size_t& k, hipEvent_t mark) {
     if (ensure_pinned(h)) return -1;
-    size_t k = 0;
     for (int g = 0; g < nseg; g++) {
         for (const auto& ch : h2d_chunks(seg[g].bytes, k == 0)) {
             const size_t off = ch.first, n = ch.second;
@@ -371,7 +400,9 @@ int h2d_staged(HostSlot& h, hipStream_t xs, const Segment* seg, int nseg) {
             if (e == hipSuccess) e = hipEventRecord(h.pin_ev[i], xs);
             if (e != hipSuccess) return set_err("hipMemcpyAsync H2D", e);
         }
-        const hipError_t e = hipEventRecord(h.up[g], xs);
+    }
+    if (mark) {
+        const hipError_t e = hipEventRecord(mark, xs);
         if (e != hipSuccess) return set_err("hipEventRecord", e);
     }
     return 0;
@@ -447,35 +478,47 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     const Segment seg[2] = {{da, (void*)in1, len1 * sizeof(u64)}, {db, (void*)in2, square ? 0 : len2 * sizeof(u64)}};
     // A second operand of at least kOverlapBytes moves on the transfer stream
     // while the first operand's forward column pass runs on the compute
-    // stream (2^20 x 2^20 limbs: 980 -> 958 us); below that the cross-stream
-    // waits cost more than they hide (2^16 x 2^16: 159 -> 170 us), and the
-    // whole call runs on the compute stream.
-    const bool overlap = !square && len2 * sizeof(u64) >= kOverlapBytes;
-    const hipStream_t xs = overlap ? h->xfer : h->stream;
-    if (h2d_staged(*h, xs, seg, 2)) die(fn);
-    {
-        std::lock_guard<std::mutex> lk(g_mu);  // the device state: twiddles, per-stream workspaces
-        hipError_t e = hipSuccess;
-        if (overlap) {
-            // the first operand's forward pass may start before the second
-            // operand has arrived (job_launch waits for up[1] where it must)
-            e = hipStreamWaitEvent(h->stream, h->up[0], 0);
-            if (e != hipSuccess) {
-                set_err("hipStreamWaitEvent", e);
-                die(fn);
-            }
+    // stream: the first half of the job is enqueued as soon as the first
+    // operand is uploaded, the rest once the second is.  Below that size the
+    // cross-stream waits cost more than they hide (2^16 x 2^16: 159 -> 170
+    // us), and the whole call runs on the compute stream.
+    const ProductIn in{da, len1, square ? nullptr : db, square ? len1 : len2, dout};
+    const bool two_streams = !square && len2 * sizeof(u64) >= kOverlapBytes;
+    size_t k = 0;
+    auto fail = [&](const char* what, hipError_t e) {
+        set_err(what, e);
+        die(fn);
+    };
+    if (two_streams) {
+        Job job;
+        Twiddles tw;
+        bool halves;
+        if (h2d_staged(*h, h->xfer, &seg[0], 1, k, h->up[0])) die(fn);
+        {
+            std::lock_guard<std::mutex> lk(g_mu);  // the device state: twiddles, per-stream workspaces
+            if (prepare_job(*c, job, &in, 1, h->stream)) die(fn);
+            tw = c->tw;
+            halves = job_parts_apart(job);  // not for a product split into pieces
+            hipError_t e = hipStreamWaitEvent(h->stream, h->up[0], 0);
+            if (e == hipSuccess && halves) e = job_launch(job, tw, h->stream, kJobFirstOperand);
+            if (e != hipSuccess) fail("kernel launch", e);
         }
-        if (run_product(*c, da, len1, db, square ? len1 : len2, square, dout, h->stream, overlap ? h->up[1] : nullptr))
-            die(fn);
-        if (overlap) {
-            e = hipEventRecord(h->done, h->stream);
-            if (e == hipSuccess) e = hipStreamWaitEvent(h->xfer, h->done, 0);
-            if (e != hipSuccess) {
-                set_err("hipEventRecord", e);
-                die(fn);
-            }
-        }
+        if (h2d_staged(*h, h->xfer, &seg[1], 1, k, h->up[1])) die(fn);
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipError_t e = hipStreamWaitEvent(h->stream, h->up[1], 0);
+        if (e == hipSuccess) e = job_launch(job, tw, h->stream, halves ? kJobAfterFirst : kJobWhole);
+        if (e == hipSuccess) e = hipEventRecord(h->done, h->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->xfer, h->done, 0);
+        if (e != hipSuccess) fail("kernel launch", e);
+    } else {
+        if (h2d_staged(*h, h->stream, seg, 2, k, nullptr)) die(fn);
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (run_job(*c, &in, 1, h->stream)) die(fn);
     }
+    const hipStream_t xs = two_streams ? h->xfer : h->stream;
+    // The operands are read, so the output (which may alias them) can be
+    // written: take its page faults while the device computes.
+    copy_pool().touch(out, out_len * sizeof(u64));
     if (d2h_staged(*h, xs, out, dout, out_len * sizeof(u64))) die(fn);
     std::lock_guard<std::mutex> lk(g_mu);
     c->free_slots.push_back(h);

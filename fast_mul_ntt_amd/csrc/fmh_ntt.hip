@@ -2013,10 +2013,10 @@ static hipError_t launch_col(ColArgs& C, const Twiddles& tw, bool fwd, bool dma,
 
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
-// wait_b (not null): the first forward pass of a multiply runs one launch per
-// operand, the second operand's after wait_b.
+// part: kJobWhole, or one of the two halves of a job that job_parts_apart()
+// accepts (the first operand's first forward pass; everything else).
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s,
-                               hipEvent_t wait_b) {
+                               JobPart part) {
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
     const Tuning tu = tuning(log_n, count >= kBatchedClass);
@@ -2039,18 +2039,14 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
-        if (pi == 0 && wait_b && nops == 2) {
-            for (int op = 0; op < 2; op++) {
-                if (op == 1) {
-                    const hipError_t we = hipStreamWaitEvent(s, wait_b, 0);
-                    if (we != hipSuccess) return we;
-                }
-                C.nops = 1;
-                C.op0 = op;
-                const bool dma = col_dma_ok(tu, C.logC, true, &j.items[first], count, 1, op);
-                const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3), 1, count), lds, s);
-                if (e != hipSuccess) return e;
-            }
+        if (pi == 0 && part != kJobWhole) {
+            // one operand: the first in the first half, the second in the other
+            C.nops = 1;
+            C.op0 = part == kJobFirstOperand ? 0 : 1;
+            const bool dma = col_dma_ok(tu, C.logC, true, &j.items[first], count, 1, C.op0);
+            const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3), 1, count), lds, s);
+            if (e != hipSuccess) return e;
+            if (part == kJobFirstOperand) return hipSuccess;
             continue;
         }
         const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &j.items[first], count, nops);
@@ -2107,20 +2103,16 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     return hipSuccess;
 }
 
-hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, hipEvent_t wait_b) {
+bool job_parts_apart(const Job& j) {
+    return j.items.size() == 1 && j.zeros.empty() && j.items[0].kind == kKindMul &&
+           make_plan((int)j.items[0].log_n, tuning((int)j.items[0].log_n)).n_col > 0;
+}
+
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, JobPart part) {
     prepare_kernels();
     if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
-    if (wait_b) {
-        // only one multiply whose transform has a column pass splits its
-        // first pass; everything else waits for both operands first
-        const bool split = j.items.size() == 1 && j.items[0].kind == kKindMul && make_plan((int)j.items[0].log_n,
-            tuning((int)j.items[0].log_n)).n_col > 0;
-        if (!split) {
-            const hipError_t we = hipStreamWaitEvent(s, wait_b, 0);
-            if (we != hipSuccess) return we;
-            wait_b = nullptr;
-        }
-    }
+    if (part != kJobWhole && !job_parts_apart(j)) return hipErrorInvalidValue;
+    if (part == kJobFirstOperand) return launch_class(j, 0, 1, tw, s, part);
     for (const auto& z : j.zeros) {
         hipError_t e = enqueue_zero(z.first, z.second, s);
         if (e != hipSuccess) return e;
@@ -2134,7 +2126,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, hipEvent_
         while (last < j.items.size() && last - first < kMaxClass && j.items[last].log_n == j.items[first].log_n &&
                j.items[last].kind == j.items[first].kind)
             last++;
-        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s, wait_b);
+        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s, part);
         if (e != hipSuccess) return e;
         first = last;
     }

@@ -70,8 +70,11 @@ def check(cond, what):
         sys.exit(1)
 
 
-# single products: single-pass and multi-pass plans, swapped operands, squares
-for n1, n2 in [(1, 1), (1, 5), (5, 1), (3, 17), (64, 64), (255, 257), (1, 2500), (2500, 1), (4097, 1)]:
+# single products: single-pass and multi-pass plans, swapped operands, squares;
+# (3000, 2500) and (2500, 2049) go through the host path that uploads the
+# second operand while the first one's forward pass runs
+for n1, n2 in [(1, 1), (1, 5), (5, 1), (3, 17), (64, 64), (255, 257), (1, 2500), (2500, 1), (4097, 1),
+               (3000, 2500), (2500, 2049)]:
     a, b = orc.splitmix(n1 * 7 + 1, n1), orc.splitmix(n2 * 11 + 2, n2)
     want = orc.mul(a, b)
     check(np.array_equal(mul(a, b), want), ("mul", n1, n2))
