@@ -45,26 +45,26 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # same runs' VALUBusy per kernel (% of one VALU instruction per SIMD every 4
 # cycles); profiles/abs_mul64_1Mx1M_kernels.md holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 179572267.0,
-    "row_mul": 157091569.0,
-    "col_inv": 160959906.0,
-    "crt_local": 70220418.0,
-    "carry_apply": 2386876.0,
+    "col_fwd": 179168151.0,
+    "row_mul": 157005717.0,
+    "col_inv": 160711867.0,
+    "crt_local": 70206554.0,
+    "carry_apply": 2386746.0,
 }
-VALU_BUSY: dict[str, float] = {"col_fwd": 78.4, "row_mul": 76.8, "col_inv": 53.3, "crt_local": 58.1, "carry_apply": 12.5}
+VALU_BUSY: dict[str, float] = {"col_fwd": 78.7, "row_mul": 76.7, "col_inv": 52.8, "crt_local": 58.0, "carry_apply": 12.3}
 PROFILE = "profiles/abs_mul64_1Mx1M_kernels.md"
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
 # the two column passes): profiles/abs_mul64_16Mx16M_kernels.md.
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3353535972.0,
-    "row_mul": 2424239940.0,
-    "col_inv": 2413008485.0,
-    "crt_local": 1121039309.0,
-    "chunk_scan": 269170.0,
-    "carry_apply": 38274278.0,
+    "col_fwd": 3365788523.0,
+    "row_mul": 2425492183.0,
+    "col_inv": 2412984831.0,
+    "crt_local": 1120991747.0,
+    "chunk_scan": 269162.0,
+    "carry_apply": 38280255.0,
 }
-VALU_BUSY_16M: dict[str, float] = {"col_fwd": 71.8, "row_mul": 86.1, "col_inv": 67.1, "crt_local": 108.9,
-                                   "chunk_scan": 0.0, "carry_apply": 20.6}
+VALU_BUSY_16M: dict[str, float] = {"col_fwd": 77.4, "row_mul": 86.0, "col_inv": 67.1, "crt_local": 107.2,
+                                   "chunk_scan": 0.0, "carry_apply": 20.4}
 PROFILE_16M = "profiles/abs_mul64_16Mx16M_kernels.md"
 
 
