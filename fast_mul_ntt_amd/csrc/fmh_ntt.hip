@@ -792,7 +792,12 @@ __device__ __forceinline__ void row_round_group(u64* __restrict__ t, int logL, i
 }
 
 // A round over the tiles of nt operands (t1 = second operand or null).
-template <int P, int R, bool DIF>
+// RMAX: that of the calling kernel instance.  This function is not inlined;
+// a copy per instance lets each be register-allocated for that instance
+// alone (shared by k_row<3> and k_row<4>, it made k_row<4> take 120 VGPRs
+// and spill; its own copy takes 109 and none: row_mul 104-107 -> 101-102 us
+// at 2^21, 1,284-1,301 -> 1,218 us at 2^25).
+template <int P, int R, bool DIF, int RMAX>
 __device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W, int nthr) {
     const int per = n >> R;
     for (int g = threadIdx.x; g < nt * per; g += nthr) {
@@ -807,10 +812,10 @@ __device__ void row_round(u64* t0, u64* t1, int nt, int n, int logL, int lq, con
 template <int P, bool DIF, int RMAX>
 __device__ void row_round_r(int r, u64* t0, u64* t1, int nt, int n, int logL, int lq, const u64* W, int nthr) {
     switch (r) {
-        case 1: row_round<P, 1, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
-        case 2: row_round<P, 2, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
-        case 3: row_round<P, 3, DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
-        default: row_round<P, (RMAX > 3 ? 4 : 3), DIF>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 1: row_round<P, 1, DIF, RMAX>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 2: row_round<P, 2, DIF, RMAX>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        case 3: row_round<P, 3, DIF, RMAX>(t0, t1, nt, n, logL, lq, W, nthr); break;
+        default: row_round<P, (RMAX > 3 ? 4 : 3), DIF, RMAX>(t0, t1, nt, n, logL, lq, W, nthr); break;
     }
 }
 
@@ -902,7 +907,7 @@ __device__ void row_round_in_r(int r, const RowLine* s_line, bool reduce, u64* t
     }
 }
 
-template <int P>
+template <int P, int RMAX>
 __device__ void row_round_out_r(int r, const RowLine* s_line, const u64* t, int n, int logL, int lq,
                                 const u64* W, int nthr) {
     switch (r) {
@@ -1120,7 +1125,7 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
     for (int lq = RF; lq < logL;) {
         const int r = logL - lq < 3 ? logL - lq : 3;
         if (lq + r == logL)
-            row_round_out_r<P>(r, s_line, tA, n, logL, lq, A.itw[P], nthr);
+            row_round_out_r<P, RMAX>(r, s_line, tA, n, logL, lq, A.itw[P], nthr);
         else
             row_round_r<P, false, RMAX>(r, tA, nullptr, 1, n, logL, lq, A.itw[P], nthr);
         lq += r;
