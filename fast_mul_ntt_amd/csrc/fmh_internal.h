@@ -69,7 +69,7 @@ struct Tuning {
 };
 // batched: the geometry for a launch that carries several products of the
 // class (table lines "b <log_n> <spec>"); single products use the plain
-// entries.
+// entries, and so do batches of a class that has no batched entry.
 Tuning tuning(int log_n, bool batched = false);
 // Applies spec to the class log_n (-1: every class); "" restores the
 // defaults of that class.  Returns 0, or -1 for a malformed spec.

@@ -139,6 +139,10 @@ static Tuning clamp_tuning(Tuning t) {
 
 static std::mutex g_tune_mu;
 static Tuning g_tune[2][kMaxLogN + 1];  // [batched][log_n]
+// Whether a batched entry was given (table line or set_tuning); a class
+// without one launches batches with its single-product geometry (the table
+// tunes batches only up to 2^17, and a longer transform fills the GPU alone).
+static bool g_tune_batched_set[kMaxLogN + 1];
 static bool g_tune_ready = false;
 static std::string g_tune_env;  // $FMH_TUNE, applied last
 
@@ -167,7 +171,10 @@ static void tuning_init_locked() {
             if (line[0] == '#' || sscanf(line + b, "%d %479s", &log_n, spec) != 2) continue;
             if (log_n < 0 || log_n > kMaxLogN) continue;
             Tuning t;
-            if (parse_tuning(spec, t)) g_tune[b][log_n] = t;
+            if (parse_tuning(spec, t)) {
+                g_tune[b][log_n] = t;
+                if (b) g_tune_batched_set[log_n] = true;
+            }
         }
         fclose(f);
     }
@@ -177,7 +184,8 @@ static void tuning_init_locked() {
 Tuning tuning(int log_n, bool batched) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tuning_init_locked();
-    Tuning t = g_tune[batched ? 1 : 0][std::max(0, std::min(log_n, kMaxLogN))];
+    const int l = std::max(0, std::min(log_n, kMaxLogN));
+    Tuning t = g_tune[batched && g_tune_batched_set[l] ? 1 : 0][l];
     if (!g_tune_env.empty()) (void)parse_tuning(g_tune_env, t);
     return clamp_tuning(t);
 }
@@ -188,9 +196,12 @@ int set_tuning(int log_n, const char* spec, bool batched) {
     tuning_init_locked();
     Tuning* tab = g_tune[batched ? 1 : 0];
     for (int l = (log_n < 0 ? 0 : log_n); l <= (log_n < 0 ? kMaxLogN : log_n); l++) {
-        Tuning t = (spec && *spec) ? tab[l] : Tuning();
+        // a batched spec without an entry of its own starts from the
+        // single-product geometry the class used until now
+        Tuning t = (spec && *spec) ? (batched && !g_tune_batched_set[l] ? g_tune[0][l] : tab[l]) : Tuning();
         if (spec && *spec && !parse_tuning(spec, t)) return -1;
         tab[l] = t;
+        if (batched) g_tune_batched_set[l] = spec && *spec;
     }
     return 0;
 }
