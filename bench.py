@@ -45,26 +45,26 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # same runs' VALUBusy per kernel (% of one VALU instruction per SIMD every 4
 # cycles); profiles/abs_mul64_1Mx1M_kernels.md holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 179168151.0,
-    "row_mul": 157005717.0,
-    "col_inv": 160711867.0,
-    "crt_local": 70206554.0,
-    "carry_apply": 2386746.0,
+    "col_fwd": 179372855.0,
+    "row_mul": 157231491.0,
+    "col_inv": 161273245.0,
+    "crt_local": 70206249.0,
+    "carry_apply": 2386733.0,
 }
-VALU_BUSY: dict[str, float] = {"col_fwd": 78.7, "row_mul": 76.7, "col_inv": 52.8, "crt_local": 58.0, "carry_apply": 12.3}
+VALU_BUSY: dict[str, float] = {"col_fwd": 78.0, "row_mul": 75.6, "col_inv": 52.8, "crt_local": 57.5, "carry_apply": 12.0}
 PROFILE = "profiles/abs_mul64_1Mx1M_kernels.md"
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
 # the two column passes): profiles/abs_mul64_16Mx16M_kernels.md.
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3365788523.0,
-    "row_mul": 2425492183.0,
-    "col_inv": 2412984831.0,
-    "crt_local": 1120991747.0,
+    "col_fwd": 3365556494.0,
+    "row_mul": 2423320842.0,
+    "col_inv": 2412965998.0,
+    "crt_local": 1120991691.0,
     "chunk_scan": 269162.0,
-    "carry_apply": 38280255.0,
+    "carry_apply": 38288447.0,
 }
-VALU_BUSY_16M: dict[str, float] = {"col_fwd": 77.4, "row_mul": 86.0, "col_inv": 67.1, "crt_local": 107.2,
-                                   "chunk_scan": 0.0, "carry_apply": 20.4}
+VALU_BUSY_16M: dict[str, float] = {"col_fwd": 77.8, "row_mul": 86.3, "col_inv": 68.1, "crt_local": 108.1,
+                                   "chunk_scan": 0.0, "carry_apply": 20.2}
 PROFILE_16M = "profiles/abs_mul64_16Mx16M_kernels.md"
 
 
