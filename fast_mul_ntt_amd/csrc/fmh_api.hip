@@ -16,6 +16,7 @@
 #include <condition_variable>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <string>
 #include <vector>
@@ -194,6 +195,9 @@ struct DevCtx {
     Twiddles tw;
     std::map<hipStream_t, Buffer> ws;
     std::vector<HostSlot*> free_slots;  // host-pointer call slots not in use
+    // compute streams of the slots in use: a call enqueues its job in two
+    // locked steps, and fmh_trim must not free the workspace in between
+    std::set<hipStream_t> slot_streams_busy;
 };
 
 constexpr int kMaxDevices = 64;
@@ -454,6 +458,7 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
         if (!c->free_slots.empty()) {
             h = c->free_slots.back();
             c->free_slots.pop_back();
+            c->slot_streams_busy.insert(h->stream);
         }
     }
     if (!h) {
@@ -466,6 +471,8 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
             set_err("host-call slot", e);
             die(fn);
         }
+        std::lock_guard<std::mutex> lk(g_mu);
+        c->slot_streams_busy.insert(h->stream);
     }
     // the slot's own buffers: no lock while data moves
     if (ensure_buffer(h->stage_a, len1 * sizeof(u64), h->stream)) die(fn);
@@ -521,6 +528,7 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     copy_pool().touch(out, out_len * sizeof(u64));
     if (d2h_staged(*h, xs, out, dout, out_len * sizeof(u64))) die(fn);
     std::lock_guard<std::mutex> lk(g_mu);
+    c->slot_streams_busy.erase(h->stream);
     c->free_slots.push_back(h);
 }
 
@@ -887,9 +895,16 @@ int fmh_trim(void) {
     DevCtx& c = g_ctx[dev];
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return set_err("hipDeviceSynchronize", e);
-    for (auto& kv : c.ws)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
-    c.ws.clear();
+    // workspaces of host-pointer calls in progress stay (such a call may
+    // enqueue the second half of its job after this returns)
+    for (auto it = c.ws.begin(); it != c.ws.end();) {
+        if (c.slot_streams_busy.count(it->first)) {
+            ++it;
+            continue;
+        }
+        if (it->second.ptr) (void)hipFree(it->second.ptr);
+        it = c.ws.erase(it);
+    }
     for (HostSlot* h : c.free_slots)  // slots in use keep their buffers
         for (Buffer* b : {&h->stage_a, &h->stage_b, &h->stage_out}) {
             if (b->ptr) (void)hipFree(b->ptr);

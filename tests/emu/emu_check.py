@@ -98,6 +98,31 @@ b = np.zeros(1030, np.uint64)
 b[0] = b[-1] = 1
 check(np.array_equal(mul(big_ones[:2100], b), orc.mul(big_ones[:2100], b)), "ones x (1 + 2^(64 1029)), 4 chunks")
 
+# fmh_trim from another thread while host-pointer multiplies run: a call that
+# enqueues its job in two halves keeps its workspace (ASan reports a freed
+# workspace as a use after free)
+import threading
+L.fmh_trim.argtypes = []
+stop_trim = threading.Event()
+
+
+def trim_loop():
+    while not stop_trim.is_set():
+        L.fmh_trim()
+
+
+tt = threading.Thread(target=trim_loop)
+tt.start()
+try:
+    a, b = orc.splitmix(61, 3000), orc.splitmix(62, 2500)
+    want = orc.mul(a, b)
+    for it in range(12):
+        check(np.array_equal(mul(a, b), want), ("mul under trim", it))
+finally:
+    stop_trim.set()
+    tt.join()
+print("trim ok", flush=True)
+
 # null pointers with nonzero lengths are refused before any launch
 L.fmh_mul_device.argtypes = [vp, u, vp, u, vp, vp]
 L.fmh_sqr_device.argtypes = [vp, u, vp, vp]
