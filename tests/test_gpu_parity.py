@@ -51,9 +51,10 @@ def test_products_vs_oracle(fm, orc, n1, n2):
 
 # Unbalanced shapes, where the long operand is cut into pieces that share one
 # spectrum of the short operand: several piece counts, a short last piece,
-# either operand the long one.
+# either operand the long one.  (1 << 22, 600000): a split product whose
+# second operand (4.8 MB) takes the drop-in's two-stream upload.
 @pytest.mark.parametrize("n1,n2", [(1 << 20, 3000), (786435, 70000), (70000, 1000), (3000, 300001),
-                                   (1 << 21, 1 << 17), (65536 * 7 + 3, 2)])
+                                   (1 << 21, 1 << 17), (65536 * 7 + 3, 2), (1 << 22, 600000)])
 def test_unbalanced_products_vs_oracle(fm, orc, n1, n2):
     a = orc.splitmix(n1 * 7 + 3, n1)
     b = orc.splitmix(n2 * 11 + 4, n2)
