@@ -1632,6 +1632,7 @@ hipError_t build_twiddles(Twiddles& tw, int log_t, hipStream_t s) {
     u64* inv_x[3] = {nullptr, nullptr, nullptr};
     std::vector<void*> made;
     auto fail = [&](hipError_t e) {
+        (void)hipStreamSynchronize(s);  // table kernels already launched finish first
         for (void* q : made) (void)hipFree(q);
         return e;
     };
