@@ -1184,6 +1184,39 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row(RowArgs A) {
     }
 }
 
+// The row-only classes of a batch (transforms of at most one row tile: every
+// line a whole product) in one launch instead of one per class.  Workgroup
+// tiles are numbered across the classes; class c owns tiles [tile0[c],
+// tile0[c + 1]) and runs as its own k_row launch would, with its own
+// arguments.  Batches of small products of many sizes (64..65,536 limbs)
+// have a dozen such classes, most of a few dozen workgroups, which one at a
+// time leave the GPU nearly idle.  The arguments of every class travel
+// whole in the argument block (14 x 264 bytes of the 4 KiB a launch takes):
+// a workgroup reads its class in place, where a copy with the class's fields
+// patched in would go to scratch memory.
+constexpr int kMaxMergedClasses = 14;
+struct RowMultiArgs {
+    u32 ncls;
+    u32 tile0[kMaxMergedClasses];
+    RowArgs cls[kMaxMergedClasses];
+};
+static_assert(sizeof(RowMultiArgs) <= 4096, "kernel argument block");
+
+__global__ __launch_bounds__(kMaxPassThreads) void k_row_small(RowMultiArgs M) {
+    __shared__ RowLine s_line[kMaxRowLines];
+    const u32 slot = xcd_slot(blockIdx.x, gridDim.x);
+    const u32 tile = slot / 3u;
+    u32 c = 0;
+    while (c + 1 < M.ncls && tile >= M.tile0[c + 1]) c++;
+    const RowArgs& A = M.cls[c];
+    const u32 t = tile - M.tile0[c];
+    switch (slot - 3u * tile) {
+        case 0: row_body<0, 4>(A, s_line, t); break;
+        case 1: row_body<1, 4>(A, s_line, t); break;
+        default: row_body<2, 4>(A, s_line, t); break;
+    }
+}
+
 // ------------------------------------------------------- CRT + carry scan
 //
 // Coefficient k of the product is c_k = CRT(x_0, x_1, x_2) in [0, P) with
@@ -1724,8 +1757,8 @@ static const CrtConsts& crt_table(int log_n) {
 // unless the limit is raised.  Launches that would exceed it are refused
 // with an error instead of being sent.
 // k_col<true, false, false>, k_col<false, false, false>, k_row<4>, k_row<3>,
-// k_col<true, true, false>, then the BIG instances of the three k_col
-static size_t g_max_dyn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+// k_col<true, true, false>, the BIG instances of the three k_col, k_row_small
+static size_t g_max_dyn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 static size_t raise_lds_limit(const void* f) {
     constexpr size_t kDefault = 64 * 1024;
@@ -1751,6 +1784,7 @@ void prepare_kernels() {
         g_max_dyn[5] = raise_lds_limit((const void*)k_col<true, false, true>);
         g_max_dyn[6] = raise_lds_limit((const void*)k_col<false, false, true>);
         g_max_dyn[7] = raise_lds_limit((const void*)k_col<true, true, true>);
+        g_max_dyn[8] = raise_lds_limit((const void*)k_row_small);
     });
 }
 
@@ -2137,15 +2171,58 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, JobPart p
     if (j.items.empty()) return hipSuccess;
     if (job_desc_bytes(j) && !j.d_desc) return hipErrorInvalidValue;
     constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
+    // Row-only classes of plain multiplies and squares in a batch go into
+    // one k_row_small launch (after the other classes: classes of these
+    // kinds do not depend on each other); classes with another row geometry
+    // than the first such class launch on their own.
+    RowMultiArgs M{};
+    size_t merged_lds = 0;
+    int merged_threads = 0;
+    u64 merged_tiles = 0;
     size_t first = 0;
     while (first < j.items.size()) {
         size_t last = first;
         while (last < j.items.size() && last - first < kMaxClass && j.items[last].log_n == j.items[first].log_n &&
                j.items[last].kind == j.items[first].kind)
             last++;
-        hipError_t e = launch_class(j, first, (u32)(last - first), tw, s, part);
-        if (e != hipSuccess) return e;
+        const u32 count = (u32)(last - first);
+        const ProductDesc& d0 = j.items[first];
+        const Tuning tu = tuning((int)d0.log_n, count >= kBatchedClass);
+        const bool small = j.items.size() > 1 && (d0.kind == kKindMul || d0.kind == kKindSqr) &&
+                           (int)d0.log_n <= tu.row_tile && tu.row_radix == 4 &&
+                           M.ncls < (u32)kMaxMergedClasses && (M.ncls == 0 || tu.row_threads == merged_threads);
+        if (small) {
+            // the row launch of launch_class, with the tiles numbered on
+            const int logL = (int)d0.log_n;
+            int logC = std::max(0, std::min(tu.row_tile - logL, 6));
+            while (logC > 0 && (1ull << (logC - 1)) >= count) logC--;
+            RowArgs& R = M.cls[M.ncls];
+            R.items = j.d_items + first;
+            R.count = count;
+            for (int i = 0; i < 3; i++) {
+                R.tw[i] = tw.fwd[i];
+                R.itw[i] = tw.inv[i];
+            }
+            R.log_n = logL;
+            R.s0 = 0;
+            R.from_limbs = 1;
+            R.mode = d0.kind == kKindMul ? kRowMul : kRowSqr;
+            R.threads = tu.row_threads;
+            R.logC = logC;
+            M.tile0[M.ncls++] = (u32)merged_tiles;
+            merged_tiles += ((u64)count + (1ull << logC) - 1) >> logC;
+            merged_lds = std::max(merged_lds, row_lds_bytes(d0.kind == kKindMul ? 2 : 1, logL, logC));
+            merged_threads = tu.row_threads;
+        } else {
+            hipError_t e = launch_class(j, first, count, tw, s, part);
+            if (e != hipSuccess) return e;
+        }
         first = last;
+    }
+    if (M.ncls) {
+        if (merged_lds > g_max_dyn[8] || merged_tiles * 3 > 0x7fffffffull) return hipErrorInvalidValue;
+        FMH_LAUNCH("row_small", k_row_small, dim3((unsigned)(merged_tiles * 3), 1, 1), dim3(merged_threads),
+                   merged_lds, s, M);
     }
     CarryArgs K{};
     K.items = j.crt.size() > 1 ? j.d_crt : nullptr;

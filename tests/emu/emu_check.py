@@ -264,6 +264,36 @@ for rep in range(2):
         check(np.array_equal(out[:n], want), ("replay", rep, i))
 check(L.fmh_batch_destroy(h) == 0, "batch destroy")
 
+# many small products and squares: the row-only classes (transforms of one
+# row tile) go into one launch, up to its class limit; the rest on their own
+rng = random.Random(6)
+items, keep = [], []
+for i in range(70):
+    n1 = rng.randint(1, 140)
+    sq = i % 3 == 0
+    n2 = n1 if sq else rng.randint(1, 140)
+    a = orc.splitmix(1200 + i, n1)
+    b = None if sq else orc.splitmix(1300 + i, n2)
+    out = np.full(n1 + n2, 7, np.uint64)
+    keep.append((a, b, out))
+    items.append(Prod(a.ctypes.data, n1, None if sq else b.ctypes.data, 0 if sq else n2, out.ctypes.data))
+arr = (Prod * len(items))(*items)
+L.fmh_timing_enable.argtypes = [ctypes.c_int]
+L.fmh_timing_collect.argtypes = [ctypes.c_char_p, u]
+L.fmh_timing_collect.restype = u
+L.fmh_timing_collect(None, 0)
+L.fmh_timing_enable(1)
+check(L.fmh_mul_batch_device(ctypes.cast(arr, vp), len(items), None) == 0, "small batch rc")
+L.fmh_timing_enable(0)
+tbuf = ctypes.create_string_buffer(4096)
+L.fmh_timing_collect(tbuf, 4096)
+launches = {ln.split()[0]: int(ln.split()[1]) for ln in tbuf.value.decode().splitlines() if ln.strip()}
+check(launches.get("row_small") == 1, ("one merged row launch", launches))
+for i, (a, b, out) in enumerate(keep):
+    want = orc.sqr(a) if b is None else orc.mul(a, b)
+    check(np.array_equal(out, want), ("small batch item", i))
+print("small batch ok", launches, flush=True)
+
 # request loop: a pool too small to hold every request at once, mixed sizes,
 # squares and spectrum requests; every output equals the oracle's
 class SrvCfg(ctypes.Structure):

@@ -285,6 +285,46 @@ def test_batch_mixed_sizes_vs_oracle(fm, orc):
             d.free()
 
 
+def test_batch_small_classes_in_one_row_launch(fm, orc):
+    """Many small products and squares (1..3000 limbs): the classes whose
+    transform is one row tile run in one merged row launch; every product
+    equals the oracle's."""
+    import random
+    rng = random.Random(4002)
+    n = 300
+    l1 = _log_uniform_sizes(rng, n, 1, 3000)
+    l2 = _log_uniform_sizes(rng, n, 1, 3000)
+    sq = [i % 4 == 0 for i in range(n)]
+    host, items, bufs = [], [], []
+    for i in range(n):
+        a = orc.splitmix(20_000 + 2 * i, l1[i])
+        b = None if sq[i] else orc.splitmix(20_001 + 2 * i, l2[i])
+        out_len = 2 * l1[i] if sq[i] else l1[i] + l2[i]
+        da = fm.DeviceBuffer.from_host(a)
+        db = None if sq[i] else fm.DeviceBuffer.from_host(b)
+        do = fm.DeviceBuffer(out_len)
+        bufs += [da, do] + ([db] if db else [])
+        items.append((da, l1[i], db, 0 if sq[i] else l2[i], do))
+        host.append((a, b, out_len))
+    try:
+        with fm.Stream() as s:
+            fm.timing_collect()
+            fm.timing_enable(True)
+            try:
+                fm.mul_batch_device(items, s)
+                s.synchronize()
+            finally:
+                fm.timing_enable(False)
+            launches = fm.timing_collect()
+            assert launches.get("row_small", (0, 0.0))[0] == 1, launches
+            for i, (a, b, out_len) in enumerate(host):
+                want = orc.sqr(a) if b is None else orc.mul(a, b)
+                assert np.array_equal(items[i][4].download(out_len), want), (i, l1[i], l2[i], sq[i])
+    finally:
+        for d in bufs:
+            d.free()
+
+
 def test_batched_table_geometry_per_class(fm, orc):
     """Batches of 5 multiplies and 1 square in each class 2^10 .. 2^17 run
     under the table's batched geometry (at least 4 products of a class):
