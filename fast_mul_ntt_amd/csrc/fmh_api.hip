@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <map>
 #include <mutex>
@@ -40,7 +41,7 @@ struct Buffer {
 // long to wake, so workers spin on the generation counter for a while after
 // each copy before they block.
 #ifndef FMH_COPY_THREADS
-#define FMH_COPY_THREADS 7
+#define FMH_COPY_THREADS 11
 #endif
 class CopyPool {
    public:
@@ -102,7 +103,7 @@ class CopyPool {
 #endif
     }
     void part(int id) {
-        const size_t per = (n_ / parts_ + 63) & ~(size_t)63;
+        const size_t per = ((n_ + parts_ - 1) / parts_ + 63) & ~(size_t)63;  // parts_ x per >= n_
         const size_t b = std::min(n_, per * (size_t)id), e = std::min(n_, b + per);
         if (e <= b) return;
         if (src_)
@@ -184,6 +185,7 @@ struct HostSlot {
     hipEvent_t up[2] = {};         // operand i is on the device
     hipEvent_t done = nullptr;     // the product is in stage_out
     Buffer stage_a, stage_b, stage_out;
+    Buffer stage_spec;             // spectrum of the first operand (two-stream calls)
     void* pin[kRing] = {};
     hipEvent_t pin_ev[kRing] = {};
 };
@@ -356,7 +358,7 @@ struct Segment {
 // transfer is cut into equal chunks of at most kPinChunk bytes (fewer DMAs:
 // each costs a few microseconds of its own).
 #ifndef FMH_PIN_HEAD
-#define FMH_PIN_HEAD (FMH_PIN_CHUNK / 4)
+#define FMH_PIN_HEAD (FMH_PIN_CHUNK / 8)
 #endif
 constexpr size_t kPinHead = FMH_PIN_HEAD;
 
@@ -484,11 +486,10 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     // both operands are read before out is written, so out may alias them
     const Segment seg[2] = {{da, (void*)in1, len1 * sizeof(u64)}, {db, (void*)in2, square ? 0 : len2 * sizeof(u64)}};
     // A second operand of at least kOverlapBytes moves on the transfer stream
-    // while the first operand's forward column pass runs on the compute
-    // stream: the first half of the job is enqueued as soon as the first
-    // operand is uploaded, the rest once the second is.  Below that size the
-    // cross-stream waits cost more than they hide (2^16 x 2^16: 159 -> 170
-    // us), and the whole call runs on the compute stream.
+    // while the device works on the first operand on the compute stream.
+    // Below that size the cross-stream waits cost more than they hide
+    // (2^16 x 2^16: 159 -> 170 us), and the whole call runs on the compute
+    // stream.
     const ProductIn in{da, len1, square ? nullptr : db, square ? len1 : len2, dout};
     const bool two_streams = !square && len2 * sizeof(u64) >= kOverlapBytes;
     size_t k = 0;
@@ -499,21 +500,46 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
     if (two_streams) {
         Job job;
         Twiddles tw;
-        bool halves;
+        // One transform with a column pass (not a product split into pieces):
+        // the first operand's whole forward transform (its resident
+        // spectrum) runs while the second operand uploads, and the rest is a
+        // product with that spectrum, which transforms only the second
+        // operand (2^20 x 2^20 limbs: the spectrum takes 88 us, under the
+        // upload; the product with it 163 us, against 243 us for the whole
+        // product).  Other products start once both operands are on the
+        // device.
+        bool spec;
+        int log_n = 0;
+        u64* d_spec = nullptr;
         if (h2d_staged(*h, h->xfer, &seg[0], 1, k, h->up[0])) die(fn);
         {
             std::lock_guard<std::mutex> lk(g_mu);  // the device state: twiddles, per-stream workspaces
-            if (prepare_job(*c, job, &in, 1, h->stream)) die(fn);
+            if (job_build(job, &in, 1)) {
+                set_err("product longer than fmh_max_ntt_len()");
+                die(fn);
+            }
+            spec = job_one_multiply_with_col(job);
+            if (spec) {
+                log_n = (int)job.items[0].log_n;
+                if (ensure_twiddles(*c, log_n)) die(fn);
+                if (ensure_buffer(h->stage_spec, (3 * sizeof(u64)) << log_n, h->stream)) die(fn);
+                d_spec = (u64*)h->stage_spec.ptr;
+            } else if (prepare_job(*c, job, &in, 1, h->stream)) {
+                die(fn);
+            }
             tw = c->tw;
-            halves = job_parts_apart(job);  // not for a product split into pieces
             hipError_t e = hipStreamWaitEvent(h->stream, h->up[0], 0);
-            if (e == hipSuccess && halves) e = job_launch(job, tw, h->stream, kJobFirstOperand);
+            if (e == hipSuccess && spec) e = enqueue_resident_spectrum(da, len1, log_n, d_spec, tw, h->stream);
             if (e != hipSuccess) fail("kernel launch", e);
         }
         if (h2d_staged(*h, h->xfer, &seg[1], 1, k, h->up[1])) die(fn);
         std::lock_guard<std::mutex> lk(g_mu);
+        if (spec) {
+            const ProductIn ins{db, len2, nullptr, len1, dout, d_spec, log_n};
+            if (prepare_job(*c, job, &ins, 1, h->stream)) die(fn);
+        }
         hipError_t e = hipStreamWaitEvent(h->stream, h->up[1], 0);
-        if (e == hipSuccess) e = job_launch(job, tw, h->stream, halves ? kJobAfterFirst : kJobWhole);
+        if (e == hipSuccess) e = job_launch(job, tw, h->stream);
         if (e == hipSuccess) e = hipEventRecord(h->done, h->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(h->xfer, h->done, 0);
         if (e != hipSuccess) fail("kernel launch", e);
@@ -906,7 +932,7 @@ int fmh_trim(void) {
         it = c.ws.erase(it);
     }
     for (HostSlot* h : c.free_slots)  // slots in use keep their buffers
-        for (Buffer* b : {&h->stage_a, &h->stage_b, &h->stage_out}) {
+        for (Buffer* b : {&h->stage_a, &h->stage_b, &h->stage_out, &h->stage_spec}) {
             if (b->ptr) (void)hipFree(b->ptr);
             *b = Buffer();
         }

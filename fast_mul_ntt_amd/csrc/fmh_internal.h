@@ -225,14 +225,11 @@ void job_pack(const Job& j, void* host);
 // Copies the descriptors with a synchronous-source hipMemcpyAsync.
 hipError_t job_upload(const Job& j, hipStream_t s);
 // Enqueues the kernels (no copies, no allocation: capturable in a graph).
-// A job of one multiply with a column pass (job_parts_apart) may be launched
-// in two halves: kJobFirstOperand (the first forward pass over the first
-// operand only) and then kJobAfterFirst (everything else), so that a
-// host-pointer call can enqueue the first half before the second operand
-// has been uploaded.
-enum JobPart { kJobWhole = 0, kJobFirstOperand = 1, kJobAfterFirst = 2 };
-bool job_parts_apart(const Job& j);
-hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, JobPart part = kJobWhole);
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s);
+// True for a job of one multiply, not split into pieces, whose transform has
+// a column pass: a host-pointer call may then compute the first operand's
+// spectrum while the second operand uploads.
+bool job_one_multiply_with_col(const Job& j);
 
 hipError_t enqueue_zero(u64* out, u64 n, hipStream_t s);
 // Plain device copy of n words (HBM calibration kernel for profilers).

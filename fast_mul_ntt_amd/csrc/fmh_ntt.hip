@@ -568,7 +568,6 @@ struct ColArgs {
     int logC;  // 2^logC adjacent lines per tile
     int from_limbs;
     int nops;     // operands transformed per product: 2 (multiply) or 1
-    int op0;      // operand of slot operand 0 (a pass over the second operand alone: 1)
     int threads;  // workgroup size
 };
 
@@ -677,7 +676,7 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_col(ColArgs A) {
     const u32 tile = slot / per_tile;
     const u32 rem = slot - tile * per_tile;
     const u32 prime = rem / (u32)A.nops;
-    const int op = A.op0 + (int)(rem - prime * (u32)A.nops);
+    const int op = (int)(rem - prime * (u32)A.nops);
     switch (prime) {
         case 0: col_body<0, FWD, DMA, BIG>(A, tile, op); break;
         case 1: col_body<1, FWD, DMA, BIG>(A, tile, op); break;
@@ -2015,12 +2014,11 @@ hipError_t job_upload(const Job& j, hipStream_t s) {
 // Whether a forward column pass over the products d[0, count) may fill its
 // tiles by LDS DMA: tiles of at least two adjacent lines, and for the first
 // pass every operand it reads (nops of each product) 16-byte aligned.
-static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const ProductDesc* d, size_t count, int nops,
-                       int op0 = 0) {
+static bool col_dma_ok(const Tuning& tu, int logC, bool from_limbs, const ProductDesc* d, size_t count, int nops) {
     if (!tu.col_dma || logC < 1) return false;
     if (!from_limbs) return true;
     for (size_t i = 0; i < count; i++)
-        for (int op = op0; op < op0 + nops; op++) {
+        for (int op = 0; op < nops; op++) {
             const u64 len = op ? d[i].nb : d[i].na;
             const uintptr_t p = (uintptr_t)(op ? d[i].b : d[i].a);
             if (len && (p & 15)) return false;
@@ -2064,10 +2062,7 @@ static hipError_t launch_col(ColArgs& C, const Twiddles& tw, bool fwd, bool dma,
 
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
-// part: kJobWhole, or one of the two halves of a job that job_parts_apart()
-// accepts (the first operand's first forward pass; everything else).
-static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s,
-                               JobPart part) {
+static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
     const ProductDesc& d0 = j.items[first];
     const int log_n = (int)d0.log_n;
     const Tuning tu = tuning(log_n, count >= kBatchedClass);
@@ -2090,16 +2085,6 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         C.logC = tu.col_logc;
         const u64 tiles = N >> (logL + C.logC);
         const size_t lds = col_lds_bytes(logL, C.logC);
-        if (pi == 0 && part != kJobWhole) {
-            // one operand: the first in the first half, the second in the other
-            C.nops = 1;
-            C.op0 = part == kJobFirstOperand ? 0 : 1;
-            const bool dma = col_dma_ok(tu, C.logC, true, &j.items[first], count, 1, C.op0);
-            const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3), 1, count), lds, s);
-            if (e != hipSuccess) return e;
-            if (part == kJobFirstOperand) return hipSuccess;
-            continue;
-        }
         const bool dma = col_dma_ok(tu, C.logC, C.from_limbs != 0, &j.items[first], count, nops);
         const hipError_t e = launch_col(C, tw, true, dma, dim3((unsigned)(tiles * 3 * nops), 1, count), lds, s);
         if (e != hipSuccess) return e;
@@ -2154,16 +2139,14 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
     return hipSuccess;
 }
 
-bool job_parts_apart(const Job& j) {
+bool job_one_multiply_with_col(const Job& j) {
     return j.items.size() == 1 && j.zeros.empty() && j.items[0].kind == kKindMul &&
            make_plan((int)j.items[0].log_n, tuning((int)j.items[0].log_n)).n_col > 0;
 }
 
-hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, JobPart part) {
+hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     prepare_kernels();
     if (j.max_log_n > tw.log_t) return hipErrorInvalidValue;
-    if (part != kJobWhole && !job_parts_apart(j)) return hipErrorInvalidValue;
-    if (part == kJobFirstOperand) return launch_class(j, 0, 1, tw, s, part);
     for (const auto& z : j.zeros) {
         hipError_t e = enqueue_zero(z.first, z.second, s);
         if (e != hipSuccess) return e;
@@ -2214,7 +2197,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s, JobPart p
             merged_lds = std::max(merged_lds, row_lds_bytes(d0.kind == kKindMul ? 2 : 1, logL, logC));
             merged_threads = tu.row_threads;
         } else {
-            hipError_t e = launch_class(j, first, count, tw, s, part);
+            hipError_t e = launch_class(j, first, count, tw, s);
             if (e != hipSuccess) return e;
         }
         first = last;
