@@ -38,9 +38,14 @@ def test_small_squares(fm, orc, n):
 
 
 # Lengths whose N = 2^n crosses every plan change: single row pass (n <= 12),
-# one column pass (13..21), two column passes (22..).
+# one column pass (13..21), two column passes (22..).  (323073, 300001): the
+# upload's second chunk (1,536,008 bytes) is not a multiple of 64 bytes per
+# copy thread, the case where a split that rounded down left its last 8 bytes
+# uncopied.  (600001, 524289): odd lengths through the two-stream upload,
+# where the first operand's spectrum is computed during the second upload.
 @pytest.mark.parametrize("n1,n2", [(2048, 2049), (4096, 4096), (3000, 9000), (16384, 16384), (32768, 32768),
-                                   (65536, 65536), (100000, 31), (262144, 262144), (1 << 20, 1 << 20)])
+                                   (65536, 65536), (100000, 31), (262144, 262144), (1 << 20, 1 << 20),
+                                   (323073, 300001), (600001, 524289)])
 def test_products_vs_oracle(fm, orc, n1, n2):
     a = orc.splitmix(n1 * 3 + 1, n1)
     b = orc.splitmix(n2 * 5 + 2, n2)
