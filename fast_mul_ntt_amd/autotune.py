@@ -19,7 +19,7 @@ single-product entry):
     col_logc     adjacent lines per column tile    4 .. 32
     col_threads  threads per column workgroup      64 .. 1024
     row_threads  threads per row workgroup         256 .. 1024
-    row_radix    stages per row-pass LDS round     3 or 4
+    row_radix    stages per row-pass LDS round     2, 3 or 4
     col_dma      forward column tiles by LDS DMA    0 or 1
 
 A candidate replaces the incumbent only if it is at least 3% faster in
@@ -54,7 +54,7 @@ CANDIDATES = {
     "col_logc": [2, 3, 4, 5],
     "col_threads": [64, 128, 256, 512, 1024],
     "row_threads": [256, 512, 1024],
-    "row_radix": [3, 4],
+    "row_radix": [2, 3, 4],
     "col_dma": [0, 1],
 }
 DEFAULT_TABLE = os.path.join(HERE, "tuning_gfx950.txt")

@@ -64,7 +64,7 @@ struct Tuning {
                             // transform of at most 2^row_tile points is one pass
     int col_threads = 512;  // threads per workgroup, column passes
     int row_threads = 512;  // threads per workgroup, row pass
-    int row_radix = 4;      // row pass: at most row_radix stages per LDS round (3 or 4)
+    int row_radix = 4;      // row pass: at most row_radix stages per LDS round (2, 3 or 4)
     int col_dma = 1;        // forward column passes: fill tiles by LDS DMA where possible (0 or 1)
 };
 // batched: the geometry for a launch that carries several products of the

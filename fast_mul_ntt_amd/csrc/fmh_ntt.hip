@@ -132,7 +132,7 @@ static Tuning clamp_tuning(Tuning t) {
     if (t.col_maxl + t.col_logc > kMaxColTileLog) t.col_maxl = kMaxColTileLog - t.col_logc;
     t.col_threads = std::max(64, std::min(t.col_threads, kMaxPassThreads)) & ~63;
     t.row_threads = std::max(64, std::min(t.row_threads, kMaxPassThreads)) & ~63;
-    t.row_radix = t.row_radix == 3 ? 3 : 4;
+    t.row_radix = t.row_radix <= 2 ? 2 : t.row_radix == 3 ? 3 : 4;
     t.col_dma = t.col_dma ? 1 : 0;
     return t;
 }
@@ -1004,7 +1004,11 @@ __device__ __forceinline__ void row_fused(const RowArgs& A, const RowLine* s_lin
 
 // RMAX: the most stages one round takes (4: radix-16 rounds for two
 // operands, more registers; 3: radix-8 rounds, fewer registers and more
-// waves per SIMD).
+// waves per SIMD; 2: radix-4 rounds, for transforms up to 2^17 points, whose
+// few workgroups run each round's code once: there the time goes to fetching
+// that code into the instruction cache (77 misses per dispatch for a
+// 128-point product with radix-16 rounds), and the short rounds execute
+// less of it: 2^7 points 20.1 -> 16.8 us, 2^13..2^17 2.5 us less).
 template <int P, int RMAX>
 __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 tile) {
     extern __shared__ u64 lds[];
@@ -1093,7 +1097,8 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
     // The first forward round reads the sources itself and the last inverse
     // round writes the residues itself; a tile with no such round is filled
     // and stored by plain copies.
-    const int RF = logL < 3 ? logL : 3;
+    constexpr int RM3 = RMAX < 3 ? RMAX : 3;  // most stages of a one-operand round and of the fused round
+    const int RF = logL < RM3 ? logL : RM3;
     const int nt = two ? 2 : 1;
     const bool fold_in = !spec_in && logL > RF;
     if (!fold_in) {
@@ -1102,7 +1107,7 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
         __syncthreads();
     }
     if (!spec_in) {
-        const int rmax = two ? RMAX : 3;
+        const int rmax = two ? RMAX : RM3;
         int top = logL;
         while (top > RF) {
             const int left = top - RF;
@@ -1133,7 +1138,7 @@ __device__ __forceinline__ void row_body(const RowArgs& A, RowLine* s_line, u32 
         return;
     }
     for (int lq = RF; lq < logL;) {
-        const int r = logL - lq < 3 ? logL - lq : 3;
+        const int r = logL - lq < RM3 ? logL - lq : RM3;
         if (lq + r == logL)
             row_round_out_r<P, RMAX>(r, s_line, tA, n, logL, lq, A.itw[P], nthr);
         else
@@ -1756,8 +1761,9 @@ static const CrtConsts& crt_table(int log_n) {
 // unless the limit is raised.  Launches that would exceed it are refused
 // with an error instead of being sent.
 // k_col<true, false, false>, k_col<false, false, false>, k_row<4>, k_row<3>,
-// k_col<true, true, false>, the BIG instances of the three k_col, k_row_small
-static size_t g_max_dyn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+// k_col<true, true, false>, the BIG instances of the three k_col, k_row_small,
+// k_row<2>
+static size_t g_max_dyn[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 static size_t raise_lds_limit(const void* f) {
     constexpr size_t kDefault = 64 * 1024;
@@ -1784,6 +1790,7 @@ void prepare_kernels() {
         g_max_dyn[6] = raise_lds_limit((const void*)k_col<false, false, true>);
         g_max_dyn[7] = raise_lds_limit((const void*)k_col<true, true, true>);
         g_max_dyn[8] = raise_lds_limit((const void*)k_row_small);
+        g_max_dyn[9] = raise_lds_limit((const void*)k_row<2>);
     });
 }
 
@@ -2060,6 +2067,22 @@ static hipError_t launch_col(ColArgs& C, const Twiddles& tw, bool fwd, bool dma,
     return hipSuccess;
 }
 
+// One row pass: the k_row instance of the geometry's round size (2, 3 or 4
+// stages per LDS round), 3 workgroups (one per prime) per tile.
+static hipError_t launch_row(int radix, const char* name, u64 tiles, int threads, size_t lds, hipStream_t s,
+                             const RowArgs& R) {
+    const int slot = radix == 2 ? 9 : radix == 3 ? 3 : 2;
+    if (lds > g_max_dyn[slot]) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(tiles * 3), 1, 1);
+    if (radix == 2)
+        FMH_LAUNCH(name, k_row<2>, grid, dim3(threads), lds, s, R);
+    else if (radix == 3)
+        FMH_LAUNCH(name, k_row<3>, grid, dim3(threads), lds, s, R);
+    else
+        FMH_LAUNCH(name, k_row<4>, grid, dim3(threads), lds, s, R);
+    return hipSuccess;
+}
+
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
@@ -2113,11 +2136,8 @@ static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twid
         R.logC = logC;
         const u64 tiles = (lines + (1ull << logC) - 1) >> logC;
         const size_t lds = row_lds_bytes(kind == kKindMul ? 2 : 1, logL, logC);
-        if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
-        if (tu.row_radix == 3)
-            FMH_LAUNCH(kRowName[kind], k_row<3>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.row_threads), lds, s, R);
-        else
-            FMH_LAUNCH(kRowName[kind], k_row<4>, dim3((unsigned)(tiles * 3), 1, 1), dim3(tu.row_threads), lds, s, R);
+        const hipError_t e = launch_row(tu.row_radix, kRowName[kind], tiles, tu.row_threads, lds, s, R);
+        if (e != hipSuccess) return e;
     }
     for (int pi = pl.n_col - 1; kind != kKindSpecFwd && pi >= 0; pi--) {
         ColArgs C{};
@@ -2156,8 +2176,10 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
     // Row-only classes of plain multiplies and squares in a batch go into
     // one k_row_small launch (after the other classes: classes of these
-    // kinds do not depend on each other); classes with another row geometry
-    // than the first such class launch on their own.
+    // kinds do not depend on each other), with radix-16 rounds whatever
+    // round size their geometry names (the products are the same); classes
+    // with another workgroup size than the first such class launch on their
+    // own.
     RowMultiArgs M{};
     size_t merged_lds = 0;
     int merged_threads = 0;
@@ -2172,7 +2194,7 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
         const ProductDesc& d0 = j.items[first];
         const Tuning tu = tuning((int)d0.log_n, count >= kBatchedClass);
         const bool small = j.items.size() > 1 && (d0.kind == kKindMul || d0.kind == kKindSqr) &&
-                           (int)d0.log_n <= tu.row_tile && tu.row_radix == 4 &&
+                           (int)d0.log_n <= tu.row_tile &&
                            M.ncls < (u32)kMaxMergedClasses && (M.ncls == 0 || tu.row_threads == merged_threads);
         if (small) {
             // the row launch of launch_class, with the tiles numbered on
@@ -2299,14 +2321,8 @@ static hipError_t enqueue_forward(const u64* a, u64 na, int log_n, u64* res, int
     R.logC = logC;
     const u64 tiles = N >> (logL + logC);
     const size_t lds = row_lds_bytes(1, logL, logC);
-    if (lds > g_max_dyn[tu.row_radix == 3 ? 3 : 2]) return hipErrorInvalidValue;
-    if (tu.row_radix == 3)
-        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<3>, dim3((unsigned)(tiles * 3), 1, 1),
-                   dim3(tu.row_threads), lds, s, R);
-    else
-        FMH_LAUNCH(mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", k_row<4>, dim3((unsigned)(tiles * 3), 1, 1),
-                   dim3(tu.row_threads), lds, s, R);
-    return hipSuccess;
+    return launch_row(tu.row_radix, mode == kRowFwdKeep ? "row_fwd_keep" : "row_fwd", tiles, tu.row_threads, lds, s,
+                      R);
 }
 
 hipError_t enqueue_spectrum(int prime, const u64* a, u64 na, int log_n, u64* out, void* ws,

@@ -88,6 +88,19 @@ for n1, n2 in [(3000, 100), (100, 3000), (5000, 7), (2600, 300)]:
     a, b = orc.splitmix(n1 + 17, n1), orc.splitmix(n2 + 19, n2)
     check(np.array_equal(mul(a, b), orc.mul(a, b)), ("unbalanced", n1, n2))
 print("unbalanced ok", flush=True)
+# rounds of at most 2 stages in the row pass (k_row<2>): single-pass and
+# multi-pass transforms, multiplies and squares
+L.fmh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+L.fmh_get_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, u]
+for lg, n1, n2 in [(7, 64, 61), (8, 100, 101), (12, 2048, 2000)]:
+    tb = ctypes.create_string_buffer(256)
+    L.fmh_get_tuning(lg, 0, tb, 256)
+    check(L.fmh_set_tuning(lg, tb.value + b",row_radix=2", 0) == 0, ("radix-2 tuning", lg))
+    a, b = orc.splitmix(lg * 5 + 1, n1), orc.splitmix(lg * 5 + 2, n2)
+    check(np.array_equal(mul(a, b), orc.mul(a, b)), ("radix-2 mul", lg))
+    check(np.array_equal(sqr(a), orc.sqr(a)), ("radix-2 sqr", lg))
+    check(L.fmh_set_tuning(lg, tb.value, 0) == 0, ("tuning restored", lg))
+print("radix-2 rows ok", flush=True)
 ones = np.full(300, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
 check(np.array_equal(mul(ones, ones[:200]), orc.mul(ones, ones[:200])), "all ones")
 # carry chains across 1024-limb chunks: runs of all-ones limbs leave the

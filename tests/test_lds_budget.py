@@ -70,9 +70,9 @@ def test_static_plus_dynamic_lds_fits_a_workgroup(tmp_path):
             static[name.group(1)] = int(size.group(1))
     col = {k: v for k, v in static.items() if "k_col" in k}
     row = {k: v for k, v in static.items() if "k_row" in k}
-    # k_col<FWD, DMA, BIG>: <true|false, false, *>, <true, true, *>; k_row<3|4>
+    # k_col<FWD, DMA, BIG>: <true|false, false, *>, <true, true, *>; k_row<2|3|4>
     # and k_row_small (the row-only classes of a batch in one launch)
-    assert len(col) == 6 and len(row) == 3, static
+    assert len(col) == 6 and len(row) == 4, static
     for k, v in col.items():
         assert v + col_max <= per_group, (k, v, col_max)
     for k, v in row.items():
