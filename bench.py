@@ -45,26 +45,26 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9
 # same runs' VALUBusy per kernel (% of one VALU instruction per SIMD every 4
 # cycles); profiles/abs_mul64_1Mx1M_kernels.md holds them.
 TRAFFIC: dict[str, float] = {
-    "col_fwd": 179372855.0,
-    "row_mul": 157231491.0,
-    "col_inv": 161273245.0,
-    "crt_local": 70206249.0,
-    "carry_apply": 2386733.0,
+    "col_fwd": 179470551.0,
+    "row_mul": 157022672.0,
+    "col_inv": 161079727.0,
+    "crt_local": 70193951.0,
+    "carry_apply": 2386741.0,
 }
-VALU_BUSY: dict[str, float] = {"col_fwd": 78.0, "row_mul": 75.6, "col_inv": 52.8, "crt_local": 57.5, "carry_apply": 12.0}
+VALU_BUSY: dict[str, float] = {"col_fwd": 78.5, "row_mul": 76.4, "col_inv": 52.8, "crt_local": 57.8, "carry_apply": 12.4}
 PROFILE = "profiles/abs_mul64_1Mx1M_kernels.md"
 # The same for the 2^24 x 2^24 product (NTT 2^25; per launch, averaged over
 # the two column passes): profiles/abs_mul64_16Mx16M_kernels.md.
 TRAFFIC_16M: dict[str, float] = {
-    "col_fwd": 3365556494.0,
-    "row_mul": 2423320842.0,
-    "col_inv": 2412965998.0,
-    "crt_local": 1120991691.0,
-    "chunk_scan": 269162.0,
-    "carry_apply": 38288447.0,
+    "col_fwd": 3352976058.0,
+    "row_mul": 2423194807.0,
+    "col_inv": 2412967116.0,
+    "crt_local": 1120974513.0,
+    "chunk_scan": 269163.0,
+    "carry_apply": 38288730.0,
 }
-VALU_BUSY_16M: dict[str, float] = {"col_fwd": 77.8, "row_mul": 86.3, "col_inv": 68.1, "crt_local": 108.1,
-                                   "chunk_scan": 0.0, "carry_apply": 20.2}
+VALU_BUSY_16M: dict[str, float] = {"col_fwd": 77.2, "row_mul": 86.9, "col_inv": 67.5, "crt_local": 109.1,
+                                   "chunk_scan": 0.0, "carry_apply": 20.3}
 PROFILE_16M = "profiles/abs_mul64_16Mx16M_kernels.md"
 
 
