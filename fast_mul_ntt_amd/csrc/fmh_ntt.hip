@@ -1221,6 +1221,19 @@ __global__ __launch_bounds__(kMaxPassThreads) void k_row_small(RowMultiArgs M) {
     }
 }
 
+// Row-only classes gathered for one k_row_small launch.
+struct MergedRows {
+    RowMultiArgs M{};
+    size_t lds = 0;
+    int threads = 0;
+    u64 tiles = 0;
+    // Takes the class's row pass (as launch_class would set it up, with the
+    // tiles numbered on); false if the class does not fit this launch.
+    bool add(const Job& j, size_t first, u32 count, const Tuning& tu, const Twiddles& tw);
+    // Launches the classes gathered so far, if any, and starts over.
+    hipError_t launch(hipStream_t s);
+};
+
 // ------------------------------------------------------- CRT + carry scan
 //
 // Coefficient k of the product is c_k = CRT(x_0, x_1, x_2) in [0, P) with
@@ -2083,6 +2096,45 @@ static hipError_t launch_row(int radix, const char* name, u64 tiles, int threads
     return hipSuccess;
 }
 
+bool MergedRows::add(const Job& j, size_t first, u32 count, const Tuning& tu, const Twiddles& tw) {
+    if (M.ncls >= (u32)kMaxMergedClasses || (M.ncls && tu.row_threads != threads)) return false;
+    const ProductDesc& d0 = j.items[first];
+    const int logL = (int)d0.log_n;
+    int logC = std::max(0, std::min(tu.row_tile - logL, 6));
+    while (logC > 0 && (1ull << (logC - 1)) >= count) logC--;
+    RowArgs& R = M.cls[M.ncls];
+    R = RowArgs{};
+    R.items = j.d_items + first;
+    R.count = count;
+    for (int i = 0; i < 3; i++) {
+        R.tw[i] = tw.fwd[i];
+        R.itw[i] = tw.inv[i];
+    }
+    R.log_n = logL;
+    R.s0 = 0;
+    R.from_limbs = 1;
+    R.mode = d0.kind == kKindMul      ? kRowMul
+             : d0.kind == kKindSqr    ? kRowSqr
+             : d0.kind == kKindMulSpec ? kRowMulSpec
+                                       : kRowFwdKeep;
+    R.threads = tu.row_threads;
+    R.logC = logC;
+    M.tile0[M.ncls++] = (u32)tiles;
+    tiles += ((u64)count + (1ull << logC) - 1) >> logC;
+    lds = std::max(lds, row_lds_bytes(d0.kind == kKindMul ? 2 : 1, logL, logC));
+    threads = tu.row_threads;
+    return true;
+}
+
+hipError_t MergedRows::launch(hipStream_t s) {
+    if (!M.ncls) return hipSuccess;
+    if (lds > g_max_dyn[8] || tiles * 3 > 0x7fffffffull) return hipErrorInvalidValue;
+    const RowMultiArgs args = M;  // a launch argument, not a member: this object starts over below
+    FMH_LAUNCH("row_small", k_row_small, dim3((unsigned)(tiles * 3), 1, 1), dim3(threads), lds, s, args);
+    *this = MergedRows();
+    return hipSuccess;
+}
+
 // Transforms of one size class: items [first, first + count) share log_n and
 // the square flag.
 static hipError_t launch_class(const Job& j, size_t first, u32 count, const Twiddles& tw, hipStream_t s) {
@@ -2174,16 +2226,16 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
     if (j.items.empty()) return hipSuccess;
     if (job_desc_bytes(j) && !j.d_desc) return hipErrorInvalidValue;
     constexpr u32 kMaxClass = 32767;  // grid.z holds the products of a class
-    // Row-only classes of plain multiplies and squares in a batch go into
-    // one k_row_small launch (after the other classes: classes of these
-    // kinds do not depend on each other), with radix-16 rounds whatever
-    // round size their geometry names (the products are the same); classes
-    // with another workgroup size than the first such class launch on their
-    // own.
-    RowMultiArgs M{};
-    size_t merged_lds = 0;
-    int merged_threads = 0;
-    u64 merged_tiles = 0;
+    // Row-only classes of a batch go into merged k_row_small launches, with
+    // radix-16 rounds whatever round size their geometry names (the
+    // products are the same): the spectra of split products (kKindSpecFwd)
+    // in one, launched before the first class of another kind (the pieces
+    // read those spectra), and the multiplies, squares and pieces
+    // (kKindMulSpec) in another after every other class (classes of these
+    // kinds do not depend on each other).  A class with another workgroup
+    // size than the first of its group, or beyond a group's 14 classes,
+    // launches on its own.
+    MergedRows spec_rows, rows;
     size_t first = 0;
     while (first < j.items.size()) {
         size_t last = first;
@@ -2193,42 +2245,25 @@ hipError_t job_launch(const Job& j, const Twiddles& tw, hipStream_t s) {
         const u32 count = (u32)(last - first);
         const ProductDesc& d0 = j.items[first];
         const Tuning tu = tuning((int)d0.log_n, count >= kBatchedClass);
-        const bool small = j.items.size() > 1 && (d0.kind == kKindMul || d0.kind == kKindSqr) &&
-                           (int)d0.log_n <= tu.row_tile &&
-                           M.ncls < (u32)kMaxMergedClasses && (M.ncls == 0 || tu.row_threads == merged_threads);
-        if (small) {
-            // the row launch of launch_class, with the tiles numbered on
-            const int logL = (int)d0.log_n;
-            int logC = std::max(0, std::min(tu.row_tile - logL, 6));
-            while (logC > 0 && (1ull << (logC - 1)) >= count) logC--;
-            RowArgs& R = M.cls[M.ncls];
-            R.items = j.d_items + first;
-            R.count = count;
-            for (int i = 0; i < 3; i++) {
-                R.tw[i] = tw.fwd[i];
-                R.itw[i] = tw.inv[i];
-            }
-            R.log_n = logL;
-            R.s0 = 0;
-            R.from_limbs = 1;
-            R.mode = d0.kind == kKindMul ? kRowMul : kRowSqr;
-            R.threads = tu.row_threads;
-            R.logC = logC;
-            M.tile0[M.ncls++] = (u32)merged_tiles;
-            merged_tiles += ((u64)count + (1ull << logC) - 1) >> logC;
-            merged_lds = std::max(merged_lds, row_lds_bytes(d0.kind == kKindMul ? 2 : 1, logL, logC));
-            merged_threads = tu.row_threads;
-        } else {
+        const bool row_only = j.items.size() > 1 && (int)d0.log_n <= tu.row_tile;
+        MergedRows* g = nullptr;
+        if (row_only && d0.kind == kKindSpecFwd)
+            g = &spec_rows;
+        else if (row_only && (d0.kind == kKindMul || d0.kind == kKindSqr || d0.kind == kKindMulSpec))
+            g = &rows;
+        if (d0.kind != kKindSpecFwd) {
+            hipError_t e = spec_rows.launch(s);
+            if (e != hipSuccess) return e;
+        }
+        if (!(g && g->add(j, first, count, tu, tw))) {
             hipError_t e = launch_class(j, first, count, tw, s);
             if (e != hipSuccess) return e;
         }
         first = last;
     }
-    if (M.ncls) {
-        if (merged_lds > g_max_dyn[8] || merged_tiles * 3 > 0x7fffffffull) return hipErrorInvalidValue;
-        FMH_LAUNCH("row_small", k_row_small, dim3((unsigned)(merged_tiles * 3), 1, 1), dim3(merged_threads),
-                   merged_lds, s, M);
-    }
+    hipError_t e = spec_rows.launch(s);
+    if (e == hipSuccess) e = rows.launch(s);
+    if (e != hipSuccess) return e;
     CarryArgs K{};
     K.items = j.crt.size() > 1 ? j.d_crt : nullptr;
     K.one = j.crt[0];

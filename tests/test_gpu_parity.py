@@ -292,8 +292,9 @@ def test_batch_mixed_sizes_vs_oracle(fm, orc):
 
 def test_batch_small_classes_in_one_row_launch(fm, orc):
     """Many small products and squares (1..3000 limbs): the classes whose
-    transform is one row tile run in one merged row launch; every product
-    equals the oracle's."""
+    transform is one row tile run in merged row launches (one for the
+    spectra of the products split into pieces, one for the rest); every
+    product equals the oracle's."""
     import random
     rng = random.Random(4002)
     n = 300
@@ -321,7 +322,7 @@ def test_batch_small_classes_in_one_row_launch(fm, orc):
             finally:
                 fm.timing_enable(False)
             launches = fm.timing_collect()
-            assert launches.get("row_small", (0, 0.0))[0] == 1, launches
+            assert launches.get("row_small", (0, 0.0))[0] in (1, 2), launches
             for i, (a, b, out_len) in enumerate(host):
                 want = orc.sqr(a) if b is None else orc.mul(a, b)
                 assert np.array_equal(items[i][4].download(out_len), want), (i, l1[i], l2[i], sq[i])
