@@ -922,7 +922,8 @@ int fmh_trim(void) {
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return set_err("hipDeviceSynchronize", e);
     // workspaces of host-pointer calls in progress stay (such a call may
-    // enqueue the second half of its job after this returns)
+    // have prepared its job on one before its second operand's upload and
+    // launch the job after this returns)
     for (auto it = c.ws.begin(); it != c.ws.end();) {
         if (c.slot_streams_busy.count(it->first)) {
             ++it;

@@ -72,7 +72,7 @@ def check(cond, what):
 
 # single products: single-pass and multi-pass plans, swapped operands, squares;
 # (3000, 2500) and (2500, 2049) go through the host path that uploads the
-# second operand while the first one's forward pass runs
+# second operand while the first one's spectrum is computed
 for n1, n2 in [(1, 1), (1, 5), (5, 1), (3, 17), (64, 64), (255, 257), (1, 2500), (2500, 1), (4097, 1),
                (3000, 2500), (2500, 2049)]:
     a, b = orc.splitmix(n1 * 7 + 1, n1), orc.splitmix(n2 * 11 + 2, n2)
@@ -112,8 +112,9 @@ b[0] = b[-1] = 1
 check(np.array_equal(mul(big_ones[:2100], b), orc.mul(big_ones[:2100], b)), "ones x (1 + 2^(64 1029)), 4 chunks")
 
 # fmh_trim from another thread while host-pointer multiplies run: a call that
-# enqueues its job in two halves keeps its workspace (ASan reports a freed
-# workspace as a use after free)
+# uses two streams works in two locked steps, before and after its second
+# operand's upload, and keeps its workspace and staging in between (ASan
+# reports a freed one as a use after free)
 import threading
 L.fmh_trim.argtypes = []
 stop_trim = threading.Event()
