@@ -416,8 +416,12 @@ int h2d_staged(HostSlot& h, hipStream_t xs, const Segment* seg, int nseg, size_t
 
 // Device -> pageable host: up to kRing chunks are in flight over DMA while
 // the host threads copy the oldest one out of its buffer; each buffer's next
-// DMA is enqueued as soon as it has been emptied.  Returns after the last
-// byte.
+// DMA is enqueued as soon as it has been emptied.  Once the first DMAs are
+// enqueued (they wait for the device's work on stream xs), the host threads
+// write one byte into every page of dst, so that a freshly allocated output
+// takes its page faults while the device computes and the first chunks move,
+// not during the copies out (dst may alias the call's inputs: they have been
+// read).  Returns after the last byte.
 int d2h_staged(HostSlot& h, hipStream_t xs, void* dst, const void* src, size_t bytes) {
     if (ensure_pinned(h)) return -1;
     const std::vector<std::pair<size_t, size_t>> ch = d2h_chunks(bytes);
@@ -430,6 +434,7 @@ int d2h_staged(HostSlot& h, hipStream_t xs, void* dst, const void* src, size_t b
     };
     hipError_t e = hipSuccess;
     for (size_t k = 0; e == hipSuccess && k < std::min(chunks, (size_t)kRing); k++) e = issue(k);
+    if (e == hipSuccess) copy_pool().touch(dst, bytes);
     for (size_t k = 0; e == hipSuccess && k < chunks; k++) {
         e = hipEventSynchronize(h.pin_ev[k % kRing]);
         if (e != hipSuccess) break;
@@ -549,9 +554,6 @@ void host_product(const char* fn, const u64* in1, u64 len1, const u64* in2, u64 
         if (run_job(*c, &in, 1, h->stream)) die(fn);
     }
     const hipStream_t xs = two_streams ? h->xfer : h->stream;
-    // The operands are read, so the output (which may alias them) can be
-    // written: take its page faults while the device computes.
-    copy_pool().touch(out, out_len * sizeof(u64));
     if (d2h_staged(*h, xs, out, dout, out_len * sizeof(u64))) die(fn);
     std::lock_guard<std::mutex> lk(g_mu);
     c->slot_streams_busy.erase(h->stream);
