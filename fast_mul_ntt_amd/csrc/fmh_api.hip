@@ -379,10 +379,27 @@ std::vector<std::pair<size_t, size_t>> h2d_chunks(size_t bytes, bool first) {
 }
 
 std::vector<std::pair<size_t, size_t>> d2h_chunks(size_t bytes) {
+    // The last chunks shrink by 2.5x down to kPinHead, so that each one's
+    // host copy finishes while the next one's DMA runs and the copy after
+    // the last DMA is short.
+    std::vector<size_t> tails;  // sizes of the last chunks, last first
+    size_t used = 0, t = kPinHead;
+    while (tails.size() < 3 && t < kPinChunk && used + t < bytes) {
+        tails.push_back(t);
+        used += t;
+        t = std::min(kPinChunk, (t * 5 / 2 + 63) & ~(size_t)63);
+    }
+    if (tails.empty() && bytes) {
+        tails.push_back(bytes);
+        used = bytes;
+    }
     std::vector<std::pair<size_t, size_t>> c;
-    const size_t tail = std::min(bytes, kPinHead);
-    equal_chunks(0, bytes - tail, c);
-    if (tail) c.push_back({bytes - tail, tail});
+    equal_chunks(0, bytes - used, c);
+    size_t off = bytes - used;
+    for (auto it = tails.rbegin(); it != tails.rend(); ++it) {
+        c.push_back({off, *it});
+        off += *it;
+    }
     return c;
 }
 
