@@ -354,9 +354,10 @@ struct Segment {
 // Chunks of a staged transfer.  Neither end of a transfer overlaps
 // anything: the first H2D chunk's host copy runs before any DMA, the last D2H
 // chunk's host copy after the last one.  So an upload starts with a chunk of
-// at most kPinHead bytes and a download ends with one; the rest of a
-// transfer is cut into equal chunks of at most kPinChunk bytes (fewer DMAs:
-// each costs a few microseconds of its own).
+// at most kPinHead bytes and a download ends with one (after a few that
+// shrink towards it); the rest of a transfer is cut into equal chunks of at
+// most kPinChunk bytes (fewer DMAs: each costs a few microseconds of its
+// own).
 #ifndef FMH_PIN_HEAD
 #define FMH_PIN_HEAD (FMH_PIN_CHUNK / 8)
 #endif
