@@ -402,9 +402,16 @@ def run_sizes(args, fm) -> dict:
             else:
                 fm.mul_device(da, n1, db, n2, do, stream)
 
-        for _ in range(args.warmup):
-            one()
-        stream.synchronize()
+        # each case warms up for at least a quarter of --min-warmup-s: after
+        # the short products the device needs that long to come back to its
+        # clocks under load (2^20 x 2^20 read 240-245 us after 10 steps)
+        w0 = time.perf_counter()
+        while True:
+            for _ in range(args.warmup):
+                one()
+            stream.synchronize()
+            if time.perf_counter() - w0 >= args.min_warmup_s / 4:
+                break
         reps = max(3, args.steps)
         t0 = time.perf_counter()
         for _ in range(reps):
