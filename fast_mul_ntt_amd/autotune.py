@@ -1,13 +1,17 @@
 """Autotuner of the NTT pass geometry, one entry per transform length.
 
-    python -m fast_mul_ntt_amd.autotune [--min-log 10] [--max-log 25] [--out PATH]
+    python -m fast_mul_ntt_amd.autotune [--min-log 5] [--max-log 25] [--out PATH]
     python -m fast_mul_ntt_amd.autotune --batched-only [--out PATH]
 
 For each size class 2^L it times a balanced product of two 2^(L-1)-limb
 operands (device-resident, seeded operands) under candidate geometries and
-keeps the fastest; for classes up to 2^17 it also tunes the geometry of
+keeps the fastest; for classes up to 2^21 it also tunes the geometry of
 batched launches, timing one batch of many such products (the table's
 "b L spec" lines, used when a launch carries several products of a class).
+Below about 2^18 points one product takes about as long on the device as
+this loop of Python calls takes to issue it, so differences of a few
+microseconds there can be hidden; the table's round sizes of 2^12..2^16
+came from a comparison driven from C.
 The search is coordinate descent over the parameters of a geometry spec,
 starting from the class's current entry of the table (with --from-defaults:
 from the built-in defaults; batched classes then start from the class's
@@ -182,9 +186,9 @@ def tune(L: int, stream: fm.Stream, reps: int, rounds: int, count: int = 1,
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--min-log", type=int, default=10)
+    ap.add_argument("--min-log", type=int, default=5)
     ap.add_argument("--max-log", type=int, default=25)
-    ap.add_argument("--max-batch-log", type=int, default=17)
+    ap.add_argument("--max-batch-log", type=int, default=21)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=DEFAULT_TABLE)
