@@ -193,16 +193,24 @@ def test_zero_length_operands(fm):
     assert fm.abs_sqr64(np.zeros(0, np.uint64)).size == 0
 
 
-def test_output_aliases_input(fm, orc):
-    """out may alias in1 (the reference consumes its inputs before writing)."""
+@pytest.mark.parametrize("n", [1000, 600000])
+def test_output_aliases_input(fm, orc, n):
+    """out may alias in1 or in2 (the reference consumes its inputs before
+    writing).  600,000 limbs: the two-stream upload, where the first
+    operand's spectrum is computed during the second upload."""
     import ctypes
-    a = orc.splitmix(5, 1000)
-    b = orc.splitmix(6, 1000)
+    P = ctypes.POINTER(ctypes.c_uint64)
+    a = orc.splitmix(5, n)
+    b = orc.splitmix(6, n)
     want = orc.mul(a, b)
-    buf = np.zeros(2000, dtype=np.uint64)
-    buf[:1000] = a
-    p = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
-    fm.lib().abs_mul64(p, 1000, b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 1000, p)
+    buf = np.zeros(2 * n, dtype=np.uint64)
+    buf[:n] = a
+    p = buf.ctypes.data_as(P)
+    fm.lib().abs_mul64(p, n, b.ctypes.data_as(P), n, p)
+    assert np.array_equal(buf, want)
+    buf[:] = 0
+    buf[n:] = b
+    fm.lib().abs_mul64(a.ctypes.data_as(P), n, buf[n:].ctypes.data_as(P), n, buf.ctypes.data_as(P))
     assert np.array_equal(buf, want)
 
 
