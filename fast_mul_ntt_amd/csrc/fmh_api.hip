@@ -358,7 +358,7 @@ struct Segment {
 // most kPinChunk bytes (fewer DMAs: each costs a few microseconds of its
 // own).
 #ifndef FMH_PIN_HEAD
-#define FMH_PIN_HEAD (FMH_PIN_CHUNK / 8)
+#define FMH_PIN_HEAD (FMH_PIN_CHUNK / 4)
 #endif
 constexpr size_t kPinHead = FMH_PIN_HEAD;
 
@@ -378,20 +378,27 @@ std::vector<std::pair<size_t, size_t>> h2d_chunks(size_t bytes, bool first) {
     return c;
 }
 
+// Downloads of at least FMH_D2H_RAMP_MIN bytes end in chunks that shrink by
+// 2.5x down to kPinHead, so that each one's host copy finishes while the
+// next one's DMA runs and the copy after the last DMA is short (16 MiB: 4.5,
+// 4.5, 5 and 2 MiB); shorter ones end in one chunk of kPinHead (4 MiB: 2 +
+// 2).  With a 1 MiB kPinHead and the ramp for every size, 2^18 x 2^18 calls
+// took 348 us against 317 us, and 2^20 x 2^20 calls 933 against 931 us.
+#ifndef FMH_D2H_RAMP_MIN
+#define FMH_D2H_RAMP_MIN (16u << 20)
+#endif
 std::vector<std::pair<size_t, size_t>> d2h_chunks(size_t bytes) {
-    // The last chunks shrink by 2.5x down to kPinHead, so that each one's
-    // host copy finishes while the next one's DMA runs and the copy after
-    // the last DMA is short.
     std::vector<size_t> tails;  // sizes of the last chunks, last first
     size_t used = 0, t = kPinHead;
-    while (tails.size() < 3 && t < kPinChunk && used + t < bytes) {
+    while (FMH_D2H_RAMP_MIN && bytes >= (size_t)FMH_D2H_RAMP_MIN && tails.size() < 3 && t < kPinChunk &&
+           used + t < bytes) {
         tails.push_back(t);
         used += t;
         t = std::min(kPinChunk, (t * 5 / 2 + 63) & ~(size_t)63);
     }
     if (tails.empty() && bytes) {
-        tails.push_back(bytes);
-        used = bytes;
+        tails.push_back(std::min(bytes, kPinHead));
+        used = tails[0];
     }
     std::vector<std::pair<size_t, size_t>> c;
     equal_chunks(0, bytes - used, c);
