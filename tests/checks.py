@@ -40,5 +40,6 @@ def primes61(count: int, seed: int) -> list[int]:
 
 
 def pool(workers: int | None = None) -> ThreadPoolExecutor:
-    """Host threads for residue checks (at most 16: the GPU box's CPU share)."""
+    """Host threads for residue checks (at most 16, whatever the host reports:
+    a machine with one GPU may give a process only 16 cores)."""
     return ThreadPoolExecutor(max_workers=workers or min(16, os.cpu_count() or 4))
